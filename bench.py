@@ -1,0 +1,139 @@
+"""Benchmark of the DiffusionLenia step engine on one MI355X.
+
+Workload: 2048 x 2048 periodic grid, 3 channels, 9 kernels (src c0=[0,1,2]*3, dst
+c1=[0,0,0,1,1,1,2,2,2]), Lenia ring kernels of radius 26, fp32, temp 1, synthetic seeded creature.
+A step is one full convolution -> growth -> diffusion -> copy pass over the grid; inputs are
+resident in HBM before the timed region.  Prints one JSON line (rank 0):
+  value      cell-steps/s (grid sites advanced per second, all channels)
+  roofline   dominant kernel: algorithmic bytes per launch / average launch time (HIP events
+             on the engine's stream), against 8 TB/s HBM
+  cpu_baseline  the C++ restatement of the reference's CPU implementation (ml-rs-example),
+             single thread, on a bounded sample of the 1024^2 C3K9 R=13 creature.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+import __graft_entry__ as ge  # noqa: E402
+
+METRIC = "cell-steps/s 2048² fp32; batched fwd+bwd grad-steps/s vs plain at 1e-5 rel err"
+HBM_PEAK_GBS = 8000.0
+KERNELS = ["row_fwd_r2c", "col_fused", "row_inv_growth", "diffusion"]
+
+
+def algorithmic_bytes_per_cell(C, K):
+    """Bytes each kernel must move per grid site (all channels), by its inputs and outputs:
+    row_fwd reads C f32 planes, writes C half spectra (N/2 complex per row = 4 B/site);
+    col_fused reads C half spectra + K kernel half spectra, writes K half spectra;
+    row_inv_growth reads K half spectra, writes C affinity planes;
+    diffusion reads C affinity + C state planes, writes C state planes."""
+    return {"row_fwd_r2c": 8 * C, "col_fused": 4 * C + 8 * K, "row_inv_growth": 4 * K + 4 * C,
+            "diffusion": 12 * C}
+
+
+def cpu_baseline(sample_steps):
+    """ml-rs-example restatement (oracle/cpu_ref.cpp, f32, one thread) on 1024^2 C3K9 R=13."""
+    orc = ge.load_oracle()
+    pkg = ge.load_package()
+    cr = pkg.baseline_creature(1024, num_channels=3, radius=13, seed=0)
+    t0 = time.perf_counter()
+    orc.run_f32(cr["state"], cr["kernels_fft"], cr["src"], cr["dst"], cr["mu"], cr["sigma"], cr["weights"],
+                steps=sample_steps)
+    dt = time.perf_counter() - t0
+    return {"value": 1024 * 1024 * sample_steps / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{sample_steps} steps of 1024x1024 C3K9 R=13 (ml-rs-example restatement, f32, "
+                      f"1 thread, {dt:.1f} s)"}
+
+
+def load_traffic(workload_tag, kernel):
+    """Measured HBM bytes per launch from committed rocprofv3 PMC summaries, if present."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload_tag}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--radius", type=int, default=26)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this engine runs on one GPU only; multi-GPU runs are not part of this build "
+              "(creature batches shard without communication, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    pkg = ge.load_package()
+    n, C, R = args.n, 3, args.radius
+    cr = pkg.baseline_creature(n, num_channels=C, radius=R, seed=0)
+    K = cr["num_kernels"]
+    eng = pkg.make_engine(cr)
+    try:
+        import torch
+        sync_dev = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is only used for a device-wide barrier
+        sync_dev = lambda: None  # noqa: E731
+
+    eng.run_steps(args.warmup)
+    eng.sync()
+    m0 = eng.mass_fp64()
+    sync_dev()
+    t0 = time.perf_counter()
+    eng.run_steps(args.steps)
+    eng.sync()
+    sync_dev()
+    dt = time.perf_counter() - t0
+    m1 = eng.mass_fp64()
+    drift = float(abs(m1.sum() - m0.sum()) / m0.sum())
+    ms_per_step = dt * 1e3 / args.steps
+    value = n * n * args.steps / dt
+
+    # Per-kernel average launch durations (HIP events on the engine stream), same number of steps.
+    prof_steps = min(args.steps, 200)
+    kms = eng.profile_kernels(prof_steps)
+    eng.close()
+    bpc = algorithmic_bytes_per_cell(C, K)
+    per_kernel = {}
+    for name, ms in zip(KERNELS, kms):
+        b = bpc[name] * n * n
+        per_kernel[name] = {"ms": round(float(ms), 5), "alg_bytes": b, "GBps": round(b / (ms * 1e-3) / 1e9, 1)}
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["ms"])
+    tag = f"{n}_c{C}k{K}"
+    achieved = per_kernel[dom]["GBps"]
+    roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(tag, dom)}
+    step_alg = sum(bpc.values()) * n * n
+    survey_model = (36 * C + 32 * K) * n * n  # C2C model: 396 B/cell-step at C3K9
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"DiffusionLenia {n}x{n} C{C}K{K} R={R} periodic, temp 1, seed 0",
+                   "grid": n, "channels": C, "kernels": K, "radius": R},
+        "roofline": roof,
+        "kernels": per_kernel,
+        "step_roofline": {"alg_bytes_per_step": step_alg, "GBps": round(step_alg / (ms_per_step * 1e-3) / 1e9, 1),
+                          "frac": round(step_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                          "c2c_model_bytes_per_step": survey_model,
+                          "c2c_model_frac": round(survey_model / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "mass_drift_rel_fp64": drift,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

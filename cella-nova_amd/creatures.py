@@ -1,0 +1,69 @@
+"""Synthetic benchmark creatures (C ABI cn_generate_creature): Lenia ring kernels of radius R with
+1-3 rings, growth mu ~ U[0.05,0.15], sigma ~ U[0.03,0.06], Dirichlet(1) weights per destination,
+U[0,1) seed on the central patch.  Deterministic in (seed, creature)."""
+import ctypes
+
+import numpy as np
+
+from ._lib import CreatureParams, check, lib
+
+
+def channel_maps(num_channels, num_kernels=None):
+    """All src x dst pairs in the reference's ordering: c0[k] = k % C (input), c1[k] = k // C
+    (output) (ml-rs/example.rs:473-477)."""
+    K = num_channels * num_channels if num_kernels is None else num_kernels
+    k = np.arange(K, dtype=np.uint32)
+    return k % num_channels, (k // num_channels) % num_channels
+
+
+def generate_creature(n, num_channels, num_kernels, src, dst, radius, patch=128, seed=0, creature=0,
+                      spectra=True, spatial=False, state=True):
+    src = np.ascontiguousarray(src, dtype=np.uint32)
+    dst = np.ascontiguousarray(dst, dtype=np.uint32)
+    rad = np.ascontiguousarray(np.broadcast_to(np.asarray(radius, dtype=np.uint32), (num_kernels,)))
+    p = CreatureParams(n, num_channels, num_kernels,
+                       src.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                       dst.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                       rad.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                       int(patch), int(seed), int(creature))
+    P = n * n
+    kf = np.empty(2 * num_kernels * P, np.float32) if spectra else None
+    sp = np.empty(num_kernels * P, np.float64) if spatial else None
+    st = np.empty(num_channels * P, np.float32) if state else None
+    mu = np.empty(num_kernels, np.float32)
+    sg = np.empty(num_kernels, np.float32)
+    w = np.empty(num_kernels, np.float32)
+
+    def ptr(a):
+        return a.ctypes.data if a is not None else None
+
+    check(lib().cn_generate_creature(ctypes.byref(p), ptr(kf), ptr(sp), ptr(st), mu.ctypes.data,
+                                     sg.ctypes.data, w.ctypes.data))
+    out = {"n": n, "num_channels": num_channels, "num_kernels": num_kernels, "src": src, "dst": dst,
+           "radius": rad, "mu": mu, "sigma": sg, "weights": w}
+    if spectra:
+        out["kernels_fft"] = kf.view(np.complex64).reshape(num_kernels, P)
+    if spatial:
+        out["spatial"] = sp.reshape(num_kernels, n, n)
+    if state:
+        out["state"] = st.reshape(num_channels, n, n)
+    return out
+
+
+def baseline_creature(n, num_channels=3, radius=13, seed=0, creature=0, patch=128, **kw):
+    """K = C^2 creature with the reference's c0/c1 maps (e.g. 1024^2 C3K9 R=13; 2048^2 R=26)."""
+    src, dst = channel_maps(num_channels)
+    return generate_creature(n, num_channels, len(src), src, dst, radius, patch=patch, seed=seed,
+                             creature=creature, **kw)
+
+
+def make_engine(cr, temp=1.0, device=0):
+    """GpuMaceLenia engine loaded with a generated creature's kernels and seed."""
+    from .engine import GpuMaceLenia
+    e = GpuMaceLenia(cr["n"], cr["num_channels"], cr["num_kernels"], cr["src"], cr["dst"], cr["mu"],
+                     cr["sigma"], cr["weights"], temp=temp, device=device)
+    for k in range(cr["num_kernels"]):
+        e.set_kernel(cr["kernels_fft"][k], k)
+    for c in range(cr["num_channels"]):
+        e.upload_channel(cr["state"][c], c)
+    return e

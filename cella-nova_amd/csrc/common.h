@@ -1,0 +1,34 @@
+// Host-side helpers shared by the C ABI translation units: thread-local error text, the
+// checked HIP-call wrapper with a fault-injection counter, and small validation utilities.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <string>
+#include "../../include/cellanova.h"
+
+namespace cn {
+
+cn_status set_error(cn_status st, const char* fmt, ...);
+void clear_error();
+
+// Returns true (and records CN_ERR_HIP) if the call failed or a failure was injected.
+bool hip_failed(hipError_t err, const char* what, cn_status* st);
+bool inject_failure();  // consumes one injected failure if any is pending
+
+#define CN_HIP(expr)                                                                  \
+  do {                                                                                \
+    cn_status _st;                                                                    \
+    if (::cn::inject_failure()) return ::cn::set_error(CN_ERR_HIP, "%s: injected failure", #expr); \
+    if (::cn::hip_failed((expr), #expr, &_st)) return _st;                            \
+  } while (0)
+
+inline bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
+inline bool finite_all(const float* p, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(p[i])) return false;
+  return true;
+}
+
+}  // namespace cn

@@ -1,0 +1,163 @@
+// Synthetic creature generator (host, FP64): Lenia ring kernels and their spectra, growth
+// parameters and seed patch for the benchmark distributions.  The reference's own generators
+// (ml-rs/example.rs:338-491, train/generate_kernel_json.py:106-215) build sigmoid-Gaussian
+// rings; the benchmark creatures use the classic Lenia shell kernel instead, defined here by
+// its closed form.  The output spectrum format is the reference's .bin layout
+// (config.rs:49-55): row-major complex64, unnormalised FFT of the origin-centred kernel.
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+#include "common.h"
+
+namespace cn {
+namespace {
+
+// Counter-based generator: splitmix64 finaliser chained over (seed, creature, stream, index).
+inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+inline double uniform01(uint64_t seed, uint64_t creature, uint64_t stream, uint64_t index) {
+  const uint64_t h = mix64(mix64(mix64(mix64(seed) + creature) + stream) + index);
+  return (double)(h >> 11) * (1.0 / 9007199254740992.0);  // [0, 1)
+}
+enum Stream : uint64_t { kRings = 1, kBeta = 2, kMu = 3, kSigma = 4, kWeight = 5, kState = 6 };
+
+// In-place radix-2 complex FFT of length n (power of two), FP64, forward (exp(-i...)).
+void fft1d(double* re, double* im, size_t n, const std::vector<double>& cs, const std::vector<double>& sn,
+           size_t table_n) {
+  for (size_t i = 1, j = 0; i < n; ++i) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) {
+      std::swap(re[i], re[j]);
+      std::swap(im[i], im[j]);
+    }
+  }
+  for (size_t len = 2; len <= n; len <<= 1) {
+    const size_t half = len / 2, step = table_n / len;
+    for (size_t i = 0; i < n; i += len)
+      for (size_t j = 0; j < half; ++j) {
+        const double wr = cs[j * step], wi = sn[j * step];
+        const double xr = re[i + j + half], xi = im[i + j + half];
+        const double vr = xr * wr - xi * wi, vi = xr * wi + xi * wr;
+        re[i + j + half] = re[i + j] - vr;
+        im[i + j + half] = im[i + j] - vi;
+        re[i + j] += vr;
+        im[i + j] += vi;
+      }
+  }
+}
+
+void fft2d(std::vector<double>& re, std::vector<double>& im, size_t n) {
+  std::vector<double> cs(n / 2 + 1), sn(n / 2 + 1);
+  for (size_t i = 0; i <= n / 2; ++i) {
+    const double a = -2.0 * M_PI * (double)i / (double)n;
+    cs[i] = std::cos(a);
+    sn[i] = std::sin(a);
+  }
+  for (size_t r = 0; r < n; ++r) fft1d(&re[r * n], &im[r * n], n, cs, sn, n);
+  std::vector<double> cr(n), ci(n);
+  for (size_t c = 0; c < n; ++c) {
+    for (size_t r = 0; r < n; ++r) {
+      cr[r] = re[r * n + c];
+      ci[r] = im[r * n + c];
+    }
+    fft1d(cr.data(), ci.data(), n, cs, sn, n);
+    for (size_t r = 0; r < n; ++r) {
+      re[r * n + c] = cr[r];
+      im[r * n + c] = ci[r];
+    }
+  }
+}
+
+}  // namespace
+}  // namespace cn
+
+using namespace cn;
+
+extern "C" cn_status cn_generate_creature(const cn_creature_params* p, float* kernels_fft, double* spatial,
+                                          float* state, float* mu, float* sigma, float* weights) {
+  clear_error();
+  if (!p) return set_error(CN_ERR_INVALID_ARG, "null params");
+  const uint32_t n = p->n, C = p->num_channels, K = p->num_kernels;
+  if (!is_pow2(n) || n < 2 || n > 8192) return set_error(CN_ERR_UNSUPPORTED, "n = %u must be a power of two", n);
+  if (C < 1 || C > 16 || K < 1 || K > 256) return set_error(CN_ERR_UNSUPPORTED, "C = %u, K = %u out of range", C, K);
+  if (!p->src || !p->dst || !p->radius) return set_error(CN_ERR_INVALID_ARG, "null src/dst/radius");
+  for (uint32_t k = 0; k < K; ++k) {
+    if (p->src[k] >= C || p->dst[k] >= C) return set_error(CN_ERR_INVALID_ARG, "src/dst[%u] out of range", k);
+    if (p->radius[k] < 1) return set_error(CN_ERR_INVALID_ARG, "radius[%u] must be >= 1", k);
+  }
+  const size_t N = n, P = N * N;
+  const uint64_t sd = p->seed, cr = p->creature;
+
+  // Growth parameters and Dirichlet(1) weights per destination channel.
+  std::vector<double> ew(K);
+  for (uint32_t k = 0; k < K; ++k) ew[k] = -std::log1p(-uniform01(sd, cr, kWeight, k));  // Exp(1)
+  for (uint32_t k = 0; k < K; ++k) {
+    double tot = 0.0;
+    for (uint32_t j = 0; j < K; ++j)
+      if (p->dst[j] == p->dst[k]) tot += ew[j];
+    if (mu) mu[k] = (float)(0.05 + 0.10 * uniform01(sd, cr, kMu, k));
+    if (sigma) sigma[k] = (float)(0.03 + 0.03 * uniform01(sd, cr, kSigma, k));
+    if (weights) weights[k] = (float)(ew[k] / tot);
+  }
+
+  // Seed: U[0,1) on the central patch of every channel.
+  if (state) {
+    const size_t ps = std::min<size_t>(p->patch, N), o = N / 2 - ps / 2;
+    std::memset(state, 0, (size_t)C * P * sizeof(float));
+    for (uint32_t c = 0; c < C; ++c)
+      for (size_t i = 0; i < ps; ++i)
+        for (size_t j = 0; j < ps; ++j)
+          state[c * P + (o + i) * N + (o + j)] = (float)uniform01(sd, cr, kState, ((uint64_t)c * ps + i) * ps + j);
+  }
+
+  if (!kernels_fft && !spatial) return CN_OK;
+  // Kernels: one worker thread per kernel (bounded by the host's hardware threads).
+  auto build = [&](uint32_t k) {
+    const double R = (double)p->radius[k];
+    const int B = 1 + (int)(3.0 * uniform01(sd, cr, kRings, k));  // U{1,2,3}
+    double beta[3];
+    for (int i = 0; i < 3; ++i) beta[i] = 0.3 + 0.7 * uniform01(sd, cr, kBeta, (uint64_t)k * 4 + i);
+    std::vector<double> re(P, 0.0), im(P, 0.0);
+    double tot = 0.0;
+    for (size_t y = 0; y < N; ++y) {
+      const double dy = (y < N / 2) ? (double)y : (double)y - (double)N;
+      for (size_t x = 0; x < N; ++x) {
+        const double dx = (x < N / 2) ? (double)x : (double)x - (double)N;
+        const double r = std::sqrt(dx * dx + dy * dy) / R;
+        if (!(r < 1.0)) continue;
+        const double s = B * r;
+        const int ri = (int)std::floor(s);
+        const double f = s - ri;
+        if (!(f > 0.0 && f < 1.0)) continue;
+        const double v = beta[ri] * std::exp(4.0 - 1.0 / (f * (1.0 - f)));
+        re[y * N + x] = v;
+        tot += v;
+      }
+    }
+    if (tot > 0.0)
+      for (size_t i = 0; i < P; ++i) re[i] /= tot;
+    if (spatial) std::memcpy(spatial + (size_t)k * P, re.data(), P * sizeof(double));
+    if (kernels_fft) {
+      fft2d(re, im, N);
+      float* o = kernels_fft + (size_t)k * P * 2;
+      for (size_t i = 0; i < P; ++i) {
+        o[2 * i] = (float)re[i];
+        o[2 * i + 1] = (float)im[i];
+      }
+    }
+  };
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  for (uint32_t k0 = 0; k0 < K; k0 += hw) {
+    std::vector<std::thread> th;
+    for (uint32_t k = k0; k < std::min(K, k0 + hw); ++k) th.emplace_back(build, k);
+    for (auto& t : th) t.join();
+  }
+  return CN_OK;
+}

@@ -1,0 +1,248 @@
+// Block-cooperative FP32 complex FFT building blocks for gfx950 (wave64).
+//
+// The transforms are self-sorting Stockham FFTs (Govindaraju et al., "High Performance
+// Discrete Fourier Transforms on Graphics Processors", SC'08) done as radix-R passes:
+// every thread holds E complex values in registers, each pass performs E/R radix-R
+// butterflies per thread in registers, and consecutive passes exchange data through LDS.
+// The first pass reads its operands from wherever the caller put them (global memory,
+// registers) and the last pass leaves its results in registers, so a caller can fuse
+// loads, pointwise products and stores into the transform without extra LDS trips.
+//
+// Index conventions for an L-point transform with radix sequence R_0, R_1, ...:
+//   threads T = L/E per transform, thread t does butterflies j = t + b*T (b < E/R_p),
+//   pass p (span Ns = R_0*...*R_{p-1}) reads x[j + m*L/R_p] (m < R_p) into v[b*R_p + m],
+//   twiddles by W_{Ns*R_p}^{m*(j mod Ns)}, and writes y[(j - j mod Ns)*R_p + j mod Ns + q*Ns].
+// The first pass therefore reads positions j + m*L/R_0 and the last pass leaves output
+// position j + q*L/R_last in v[b*R_last + q].
+//
+// The forward plan uses radix E for every pass except a final remainder pass; the inverse
+// plan puts the remainder first.  So the forward transform's output register layout is
+// exactly the inverse transform's input register layout, which is what lets a fused
+// "forward FFT -> pointwise product -> inverse FFT" keep the spectrum in registers.
+//
+// Twiddles come from a table tw[i] = exp(-2*pi*i*i/TWN) computed in FP64 on the host and
+// rounded once to FP32; an L-point transform uses stride TWN/L into it.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace cn {
+
+__device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 c_mul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 c_mulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 c_conj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 c_scale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
+
+// cos(2*pi*k/16), sin(2*pi*k/16) for k = 0..3 (enough for every radix-16 sub-twiddle
+// after the special angles 0, 1/8, 1/4, 3/8 are handled exactly).
+constexpr float kC16_1 = 0.92387953251128675613f;  // cos(pi/8)
+constexpr float kS16_1 = 0.38268343236508977173f;  // sin(pi/8)
+constexpr float kSqrtHalf = 0.70710678118654752440f;
+constexpr float kC32_1 = 0.98078528040323044913f;  // cos(pi/16)
+constexpr float kS32_1 = 0.19509032201612826785f;  // sin(pi/16)
+constexpr float kC32_3 = 0.83146961230254523708f;  // cos(3pi/16)
+constexpr float kS32_3 = 0.55557023301960222474f;  // sin(3pi/16)
+
+// v * W_R^q with W_R = exp(-2*pi*i/R) (forward) or its conjugate (inverse), q < R/2.
+// q and R are compile-time constants after unrolling, so the branches fold away.
+template <int R, bool INV>
+__device__ __forceinline__ float2 mul_wq(float2 v, int q) {
+  // Reduce q/R to an angle in units of 1/32 turn: a = 32*q/R (exact for R <= 32).
+  const int a = (32 / R) * q;  // 0..15
+  float c, s;                   // W = c - i*s for forward
+  if (a == 0) return v;
+  if (a == 8) return INV ? make_float2(-v.y, v.x) : make_float2(v.y, -v.x);
+  if (a == 4) {
+    return INV ? make_float2((v.x - v.y) * kSqrtHalf, (v.x + v.y) * kSqrtHalf)
+               : make_float2((v.x + v.y) * kSqrtHalf, (v.y - v.x) * kSqrtHalf);
+  }
+  if (a == 12) {
+    return INV ? make_float2(-(v.x + v.y) * kSqrtHalf, (v.x - v.y) * kSqrtHalf)
+               : make_float2((v.y - v.x) * kSqrtHalf, -(v.x + v.y) * kSqrtHalf);
+  }
+  switch (a) {
+    case 2: c = kC16_1; s = kS16_1; break;
+    case 6: c = kS16_1; s = kC16_1; break;
+    case 10: c = -kS16_1; s = kC16_1; break;
+    case 14: c = -kC16_1; s = kS16_1; break;
+    case 1: c = kC32_1; s = kS32_1; break;
+    case 3: c = kC32_3; s = kS32_3; break;
+    case 5: c = kS32_3; s = kC32_3; break;
+    case 7: c = kS32_1; s = kC32_1; break;
+    case 9: c = -kS32_1; s = kC32_1; break;
+    case 11: c = -kS32_3; s = kC32_3; break;
+    case 13: c = -kC32_3; s = kS32_3; break;
+    default: c = -kC32_1; s = kS32_1; break;  // 15
+  }
+  // forward: v * (c - i s); inverse: v * (c + i s)
+  return INV ? make_float2(v.x * c - v.y * s, v.y * c + v.x * s)
+             : make_float2(v.x * c + v.y * s, v.y * c - v.x * s);
+}
+
+// In-register R-point DFT (R <= 32), natural order in and out, radix-2 decimation in time.
+template <int R, bool INV>
+struct Dft {
+  static __device__ __forceinline__ void run(float2* a) {
+    float2 e[R / 2], o[R / 2];
+#pragma unroll
+    for (int i = 0; i < R / 2; ++i) {
+      e[i] = a[2 * i];
+      o[i] = a[2 * i + 1];
+    }
+    Dft<R / 2, INV>::run(e);
+    Dft<R / 2, INV>::run(o);
+#pragma unroll
+    for (int q = 0; q < R / 2; ++q) {
+      const float2 t = mul_wq<R, INV>(o[q], q);
+      a[q] = c_add(e[q], t);
+      a[q + R / 2] = c_sub(e[q], t);
+    }
+  }
+};
+template <bool INV>
+struct Dft<1, INV> {
+  static __device__ __forceinline__ void run(float2*) {}
+};
+template <bool INV>
+struct Dft<2, INV> {
+  static __device__ __forceinline__ void run(float2* a) {
+    const float2 x = a[0], y = a[1];
+    a[0] = c_add(x, y);
+    a[1] = c_sub(x, y);
+  }
+};
+template <bool INV>
+struct Dft<4, INV> {
+  static __device__ __forceinline__ void run(float2* a) {
+    const float2 t0 = c_add(a[0], a[2]), t1 = c_sub(a[0], a[2]);
+    const float2 t2 = c_add(a[1], a[3]), d = c_sub(a[1], a[3]);
+    const float2 t3 = INV ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);
+    a[0] = c_add(t0, t2);
+    a[2] = c_sub(t0, t2);
+    a[1] = c_add(t1, t3);
+    a[3] = c_sub(t1, t3);
+  }
+};
+
+// Pass plan of an L-point transform with E values per thread.
+template <int L, int E>
+struct Plan {
+  static_assert(E >= 2 && (E & (E - 1)) == 0, "E must be a power of two");
+  static_assert(L >= E && (L & (L - 1)) == 0, "L must be a power of two >= E");
+  static constexpr int T = L / E;
+  static constexpr int lgL = ilog2c(L), lgE = ilog2c(E);
+  static constexpr int nfull = lgL / lgE;
+  static constexpr int rem = 1 << (lgL - nfull * lgE);
+  static constexpr int npass = nfull + (rem > 1 ? 1 : 0);
+  static constexpr int radix(int p, bool inv) {
+    return rem == 1 ? E : (inv ? (p == 0 ? rem : E) : (p == npass - 1 ? rem : E));
+  }
+  static constexpr int span(int p, bool inv) {
+    int s = 1;
+    for (int i = 0; i < p; ++i) s *= radix(i, inv);
+    return s;
+  }
+  static constexpr int first_radix(bool inv) { return radix(0, inv); }
+  static constexpr int last_radix(bool inv) { return radix(npass - 1, inv); }
+};
+
+// LDS addressing of element `pos` of transform `lane` (0 <= lane < NT) in a block that holds
+// NT independent transforms.  INTERLEAVED: pos*NT + lane (lane-consecutive threads touch
+// consecutive addresses; used for column tiles).  Otherwise lane*(L+PAD) + pos.
+template <int L, int NT, bool INTERLEAVED>
+struct LdsMap {
+  static constexpr int PAD = INTERLEAVED ? 0 : 0;
+  static constexpr int size = INTERLEAVED ? L * NT : NT * (L + PAD);
+  static __device__ __forceinline__ int at(int pos, int lane) {
+    return INTERLEAVED ? pos * NT + lane : lane * (L + PAD) + pos;
+  }
+};
+
+// One Stockham pass on register data (twiddle + radix-R butterflies) for thread t.
+template <int L, int E, int R, int NS, bool INV, int TWN>
+__device__ __forceinline__ void pass_compute(float2* v, int t, const float2* __restrict__ tw) {
+  constexpr int T = L / E;
+  constexpr int NB = E / R;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + b * T;
+    if constexpr (NS > 1) {
+      const int k = j & (NS - 1);
+      constexpr int step = (TWN / L) * (L / (NS * R));  // table stride for W_{NS*R}
+#pragma unroll
+      for (int m = 1; m < R; ++m) {
+        const float2 w = tw[m * k * step];
+        v[b * R + m] = INV ? c_mulc(v[b * R + m], w) : c_mul(v[b * R + m], w);
+      }
+    }
+    Dft<R, INV>::run(v + b * R);
+  }
+}
+
+// Hides the value of a lane index from the optimiser.  Index arithmetic derived from a
+// laundered value is recomputed where it is used instead of being hoisted out of the
+// caller's loops as dozens of loop-invariant 64-bit addresses (which spill).
+__device__ __forceinline__ int launder(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+// Runs passes P..npass-1 of an L-point transform.  On entry v holds the inputs of pass P
+// (layout of pass P); on exit it holds the outputs of the last pass.  `lds` must hold
+// LdsMap<L,NT,IL>::size elements; every exchange is bracketed by block barriers, so the
+// routine can be called repeatedly on the same buffer.
+template <int L, int E, int NT, bool IL, bool INV, int TWN, int P = 0>
+__device__ __forceinline__ void fft_run(float2* v, float2* lds, int t, int lane,
+                                        const float2* __restrict__ tw) {
+  using PL = Plan<L, E>;
+  constexpr int R = PL::radix(P, INV);
+  constexpr int NS = PL::span(P, INV);
+  t = launder(t);
+  pass_compute<L, E, R, NS, INV, TWN>(v, t, tw);
+  if constexpr (P + 1 < PL::npass) {
+    constexpr int T = L / E;
+    constexpr int R2 = PL::radix(P + 1, INV);
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < E / R; ++b) {
+      const int j = t + b * T;
+      const int k = j & (NS - 1);
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int q = 0; q < R; ++q) lds[LdsMap<L, NT, IL>::at(base + q * NS, lane)] = v[b * R + q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < E / R2; ++b) {
+      const int j = t + b * T;
+#pragma unroll
+      for (int m = 0; m < R2; ++m) v[b * R2 + m] = lds[LdsMap<L, NT, IL>::at(j + m * (L / R2), lane)];
+    }
+    fft_run<L, E, NT, IL, INV, TWN, P + 1>(v, lds, t, lane, tw);
+  }
+}
+
+// Position (within the transform) of register slot i for the first pass's inputs and the
+// last pass's outputs.
+template <int L, int E, bool INV>
+__device__ __forceinline__ int in_pos(int t, int i) {
+  using PL = Plan<L, E>;
+  constexpr int R = PL::first_radix(INV);
+  return t + (i / R) * PL::T + (i % R) * (L / R);
+}
+template <int L, int E, bool INV>
+__device__ __forceinline__ int out_pos(int t, int i) {
+  using PL = Plan<L, E>;
+  constexpr int R = PL::last_radix(INV);
+  return t + (i / R) * PL::T + (i % R) * (L / R);
+}
+
+}  // namespace cn

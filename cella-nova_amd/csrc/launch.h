@@ -1,0 +1,25 @@
+// Host-side launch interface of the device kernels (shared by the engine and the batch path).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "types.h"
+
+namespace cn {
+
+struct StepLaunch {
+  const CreatureDev* descs = nullptr;  // device array of `count` descriptors, all with the same N
+  int count = 0;
+  int parity = 0;                      // state[parity] -> state[parity ^ 1]
+  int max_c = 0;                       // max channels over the descriptors
+  const float2* tw = nullptr;          // kTwN-entry twiddle table
+  bool write_u = false;                // also write the real convolution results to desc.u
+  bool skip_diffusion = false;         // stop after the affinity (conv readback)
+  // Optional per-kernel timing events: ev[0] before the row pass, ev[1] after it, ev[2] after
+  // the column pass, ev[3] after the inverse-row + growth pass, ev[4] after the diffusion.
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+};
+
+hipError_t launch_step(int N, const StepLaunch& sl, hipStream_t st);
+hipError_t launch_mass(const CreatureDev* descs, int count, int parity, int max_c, double2* partial, int nblk,
+                       hipStream_t st);
+
+}  // namespace cn

@@ -1,0 +1,381 @@
+// Forward DiffusionLenia step kernels for gfx950.
+//
+// One step (reference phase sequence convolution -> growth -> diffusion -> copy,
+// ml-rs/orchestrator/mod.rs:348-386) is four launches per grid-size class:
+//   k_row_fwd         real rows -> half spectra (R2C via an N/2-point complex FFT)
+//   k_col_fused       per source channel and column tile: forward column FFT, then for every
+//                     kernel reading that source: product with the kernel spectrum and inverse
+//                     column FFT (the spectrum stays in registers between kernels)
+//   k_row_inv_growth  per row: C2R inverse row FFT of every kernel's result, growth bump and
+//                     per-destination weighted sum, affinity exp (growth.rs, compute_1024.wgsl:299-325)
+//   k_diffusion       halo-tiled 3x3 Z sum and mass-conserving gather (compute_1024.wgsl:341-413)
+// The copy phase is a pointer swap (ping-pong state).
+#include <hip/hip_runtime.h>
+#include "fft.h"
+#include "types.h"
+#include "launch.h"
+
+namespace cn {
+
+template <int N>
+struct Cfg {
+  static constexpr int L_ROW = N / 2;
+  static constexpr int E_ROW = L_ROW >= 16 ? 16 : L_ROW;
+  static constexpr int T_ROW = L_ROW / E_ROW;
+  static constexpr int ROWS = (256 / T_ROW) < N ? (256 / T_ROW) : N;
+  static constexpr int ROW_THREADS = ROWS * T_ROW;
+  static constexpr int E_COL = N >= 16 ? 16 : N;
+  static constexpr int T_COL = N / E_COL;
+  // column tile width (complex columns per block of the fused column pass)
+  static constexpr int TC_DEF = N >= 2048 ? 4 : (N >= 1024 ? 8 : 16);
+  static constexpr int TC = TC_DEF < N / 2 ? TC_DEF : N / 2;
+};
+
+// ---------------------------------------------------------------------------------------
+// Row forward: state rows (real, N) -> half spectrum rows (N/2 complex, slot 0 = (X0, X_{N/2}))
+// ---------------------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(Cfg<N>::ROW_THREADS)
+k_row_fwd(const CreatureDev* __restrict__ descs, int parity, const float2* __restrict__ tw) {
+  using CF = Cfg<N>;
+  constexpr int L = CF::L_ROW, E = CF::E_ROW, T = CF::T_ROW, ROWS = CF::ROWS;
+  const CreatureDev& d = descs[blockIdx.z];
+  if (d.N != N) return;
+  const int nrows = d.C * N;
+  const int row0 = blockIdx.x * ROWS;
+  if (row0 >= nrows) return;
+  const int lane = threadIdx.x / T, t = threadIdx.x % T;
+  int row = row0 + lane;
+  const bool valid = row < nrows;
+  if (!valid) row = nrows - 1;
+
+  const float2* in = reinterpret_cast<const float2*>(d.state[parity] + (size_t)row * N);
+  float2 v[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) v[i] = in[in_pos<L, E, false>(t, i)];
+
+  __shared__ float2 lds[LdsMap<L, ROWS, false>::size];
+  fft_run<L, E, ROWS, false, false, kTwN>(v, lds, t, lane, tw);
+
+  // Split the packed transform Z of z[n] = x[2n] + i x[2n+1] into the real row's spectrum.
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < E; ++i) lds[LdsMap<L, ROWS, false>::at(out_pos<L, E, false>(t, i), lane)] = v[i];
+  __syncthreads();
+  float2* out = reinterpret_cast<float2*>(d.spec) + (size_t)row * L;
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int p = out_pos<L, E, false>(t, i);
+    const float2 zp = v[i];
+    float2 X;
+    if (p == 0) {
+      X = make_float2(zp.x + zp.y, zp.x - zp.y);
+    } else {
+      const float2 zq = lds[LdsMap<L, ROWS, false>::at(L - p, lane)];
+      const float2 ze = make_float2(0.5f * (zp.x + zq.x), 0.5f * (zp.y - zq.y));
+      const float2 dd = make_float2(zp.x - zq.x, zp.y + zq.y);  // zp - conj(zq)
+      const float2 zo = make_float2(0.5f * dd.y, -0.5f * dd.x);  // dd / (2i)
+      X = c_add(ze, c_mul(tw[p * (kTwN / N)], zo));
+    }
+    if (valid) out[p] = X;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Fused column pass: forward column FFT of one source tile, then per kernel of that source:
+// product with the kernel's half spectrum and inverse column FFT.
+// ---------------------------------------------------------------------------------------
+template <int N, int TC>
+__global__ void __launch_bounds__(TC * Cfg<N>::T_COL) __attribute__((amdgpu_waves_per_eu(4)))
+k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) {
+  using CF = Cfg<N>;
+  constexpr int L = N, E = CF::E_COL, HALF = N / 2;
+  const CreatureDev& d = descs[blockIdx.z];
+  if (d.N != N) return;
+  const int c = blockIdx.y;
+  if (c >= d.C) return;
+  const int kb = d.src_off[c], ke = d.src_off[c + 1];
+  if (kb == ke) return;
+  const int x0 = blockIdx.x * TC;
+  const int lane = threadIdx.x % TC;
+  int t = threadIdx.x / TC;
+  const int x = x0 + lane;
+
+  const float2* sp = reinterpret_cast<const float2*>(d.spec) + (size_t)c * N * HALF;
+  float2 v[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) v[i] = sp[(size_t)in_pos<L, E, false>(t, i) * HALF + x];
+
+  __shared__ float2 lds[L * TC];
+  __shared__ float2 ldsB[N];
+  fft_run<L, E, TC, true, false, kTwN>(v, lds, t, lane, tw);
+
+  // Tile 0, column 0 carries two real columns packed as (DC column) + i (Nyquist column):
+  // split F into A = FFT(DC column) and B = FFT(Nyquist column) once per source.
+  const bool edge = (x0 == 0);
+  if (edge) {
+    if (lane == 0) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) ldsB[out_pos<L, E, false>(t, i)] = v[i];
+    }
+    __syncthreads();
+    float2 bb[E];
+    if (lane == 0) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const int y = out_pos<L, E, false>(t, i);
+        const float2 F = v[i];
+        const float2 G = ldsB[(N - y) & (N - 1)];
+        const float2 a = make_float2(0.5f * (F.x + G.x), 0.5f * (F.y - G.y));   // (F + conj G)/2
+        const float2 dd = make_float2(F.x - G.x, F.y + G.y);                     // F - conj G
+        bb[i] = make_float2(0.5f * dd.y, -0.5f * dd.x);                          // dd / (2i)
+        v[i] = a;
+      }
+    }
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) ldsB[out_pos<L, E, false>(t, i)] = bb[i];
+    }
+    __syncthreads();
+  }
+
+  for (int kk = kb; kk < ke; ++kk) {
+    const int k = d.src_k[kk];
+    t = launder(t);
+    const float2* kh = reinterpret_cast<const float2*>(d.khalf) + (size_t)k * N * HALF;
+    float2 w[E];
+    if (edge && lane == 0) {
+      const float2* kn = reinterpret_cast<const float2*>(d.knyq) + (size_t)k * N;
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const int y = out_pos<L, E, false>(t, i);
+        const float2 pa = c_mul(v[i], kh[(size_t)y * HALF]);
+        const float2 pb = c_mul(ldsB[y], kn[y]);
+        w[i] = make_float2(pa.x - pb.y, pa.y + pb.x);  // pa + i*pb
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const int y = out_pos<L, E, false>(t, i);
+        w[i] = c_mul(v[i], kh[(size_t)y * HALF + x]);
+      }
+    }
+    fft_run<L, E, TC, true, true, kTwN>(w, lds, t, lane, tw);
+    float2* uo = reinterpret_cast<float2*>(d.uhat) + (size_t)k * N * HALF;
+#pragma unroll
+    for (int i = 0; i < E; ++i) uo[(size_t)out_pos<L, E, true>(t, i) * HALF + x] = w[i];
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Row inverse (C2R) of every kernel's result + growth + weighted sum + affinity exp.
+// ---------------------------------------------------------------------------------------
+template <int N, bool WRITE_U>
+__global__ void __launch_bounds__(Cfg<N>::ROW_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
+k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) {
+  using CF = Cfg<N>;
+  constexpr int L = CF::L_ROW, E = CF::E_ROW, T = CF::T_ROW, ROWS = CF::ROWS;
+  const CreatureDev& d = descs[blockIdx.z];
+  if (d.N != N) return;
+  const int lane = threadIdx.x / T, t0 = threadIdx.x % T;
+  const int y = blockIdx.x * ROWS + lane;  // ROWS divides N
+  __shared__ float2 lds[LdsMap<L, ROWS, false>::size];
+  const float temp = d.temp;
+
+  for (int c = 0; c < d.C; ++c) {
+    float2 acc[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) acc[i] = make_float2(0.f, 0.f);
+    const int kb = d.dst_off[c], ke = d.dst_off[c + 1];
+    for (int kk = kb; kk < ke; ++kk) {
+      const int k = d.dst_k[kk];
+      const int t = launder(t0);
+      const float mu = d.mu[k], is2 = d.inv_s2[k], wk = d.w[k];
+      const float2* X = reinterpret_cast<const float2*>(d.uhat) + ((size_t)k * N + y) * L;
+      float2 v[E];
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const int p = in_pos<L, E, true>(t, i);
+        const float2 xp = X[p];
+        if (p == 0) {
+          v[i] = make_float2(xp.x + xp.y, xp.x - xp.y);
+        } else {
+          const float2 xq = X[L - p];
+          const float2 e2 = make_float2(xp.x + xq.x, xp.y - xq.y);       // xp + conj(xq)
+          const float2 dd = make_float2(xp.x - xq.x, xp.y + xq.y);       // xp - conj(xq)
+          const float2 o2 = c_mulc(dd, tw[p * (kTwN / N)]);               // * W_N^{-p}
+          v[i] = make_float2(e2.x - o2.y, e2.y + o2.x);                  // e2 + i*o2
+        }
+      }
+      fft_run<L, E, ROWS, false, true, kTwN>(v, lds, t, lane, tw);
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const int n = out_pos<L, E, true>(t, i);
+        const float u0 = v[i].x, u1 = v[i].y;  // u[2n], u[2n+1]
+        if constexpr (WRITE_U) {
+          reinterpret_cast<float2*>(d.u + ((size_t)k * N + y) * N)[n] = v[i];
+        }
+        const float d0 = u0 - mu, d1 = u1 - mu;
+        const float g0 = 2.0f * expf(-0.5f * d0 * d0 * is2) - 1.0f;
+        const float g1 = 2.0f * expf(-0.5f * d1 * d1 * is2) - 1.0f;
+        acc[i].x += g0 * wk;
+        acc[i].y += g1 * wk;
+      }
+    }
+    float2* Ao = reinterpret_cast<float2*>(d.A + ((size_t)c * N + y) * N);
+    const int t = launder(t0);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int n = out_pos<L, E, true>(t, i);
+      Ao[n] = make_float2(expf(temp * acc[i].x), expf(temp * acc[i].y));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Diffusion: Z = box3(A) (periodic), new = A * box3(state / Z).  Halo-2 tile in LDS.
+// ---------------------------------------------------------------------------------------
+constexpr int kDiffTW = 64, kDiffTH = 16, kDiffThreads = 256;
+
+__global__ void __launch_bounds__(kDiffThreads)
+k_diffusion(const CreatureDev* __restrict__ descs, int parity, int N) {
+  constexpr int TW = kDiffTW, TH = kDiffTH;
+  const CreatureDev& d = descs[blockIdx.z];
+  if (d.N != N) return;
+  const int tiles_y = (N + TH - 1) / TH;
+  const int c = blockIdx.y / tiles_y;
+  if (c >= d.C) return;
+  const int y0 = (blockIdx.y % tiles_y) * TH;
+  const int x0 = blockIdx.x * TW;
+  const int mask = N - 1;
+  const size_t plane = (size_t)N * N;
+  const float* A = d.A + c * plane;
+  const float* s = d.state[parity] + c * plane;
+  float* o = d.state[parity ^ 1] + c * plane;
+
+  __shared__ float sA[TH + 4][TW + 5];
+  __shared__ float sQ[TH + 2][TW + 3];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < (TH + 4) * (TW + 4); i += kDiffThreads) {
+    const int r = i / (TW + 4), cc = i % (TW + 4);
+    const int gy = (y0 - 2 + r) & mask, gx = (x0 - 2 + cc) & mask;
+    sA[r][cc] = A[(size_t)gy * N + gx];
+  }
+  __syncthreads();
+  for (int i = tid; i < (TH + 2) * (TW + 2); i += kDiffThreads) {
+    const int r = i / (TW + 2), cc = i % (TW + 2);
+    float z = 0.f;
+#pragma unroll
+    for (int dr = 0; dr < 3; ++dr)
+#pragma unroll
+      for (int dc = 0; dc < 3; ++dc) z += sA[r + dr][cc + dc];
+    const int gy = (y0 - 1 + r) & mask, gx = (x0 - 1 + cc) & mask;
+    sQ[r][cc] = s[(size_t)gy * N + gx] / z;
+  }
+  __syncthreads();
+  for (int i = tid; i < TH * TW; i += kDiffThreads) {
+    const int r = i / TW, cc = i % TW;
+    const int gy = y0 + r, gx = x0 + cc;
+    float acc = 0.f;
+#pragma unroll
+    for (int dr = 0; dr < 3; ++dr)
+#pragma unroll
+      for (int dc = 0; dc < 3; ++dc) acc += sQ[r + dr][cc + dc];
+    if (gy < N && gx < N) o[(size_t)gy * N + gx] = sA[r + 2][cc + 2] * acc;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// FP64 Neumaier-compensated mass per channel (the only FP64 arithmetic on the device).
+// partial[(creature * 16 + c) * nblk + blockIdx.x] = (sum, compensation)
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void neumaier_add(double& s, double& comp, double x) {
+  const double t = s + x;
+  if (fabs(s) >= fabs(x)) comp += (s - t) + x;
+  else comp += (x - t) + s;
+  s = t;
+}
+
+constexpr int kMassThreads = 256;
+
+__global__ void __launch_bounds__(kMassThreads)
+k_mass(const CreatureDev* __restrict__ descs, int parity, double2* __restrict__ partial, int nblk) {
+  const CreatureDev& d = descs[blockIdx.z];
+  const int c = blockIdx.y;
+  double2* dst = partial + ((size_t)blockIdx.z * kMaxChannels + c) * nblk + blockIdx.x;
+  if (c >= d.C) return;
+  const size_t plane = (size_t)d.N * d.N;
+  const float* s = d.state[parity] + c * plane;
+  double sum = 0.0, comp = 0.0;
+  for (size_t i = (size_t)blockIdx.x * kMassThreads + threadIdx.x; i < plane; i += (size_t)nblk * kMassThreads)
+    neumaier_add(sum, comp, (double)s[i]);
+  __shared__ double ss[kMassThreads], sc[kMassThreads];
+  ss[threadIdx.x] = sum;
+  sc[threadIdx.x] = comp;
+  __syncthreads();
+  for (int h = kMassThreads / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      double a = ss[threadIdx.x], ca = sc[threadIdx.x] + sc[threadIdx.x + h];
+      neumaier_add(a, ca, ss[threadIdx.x + h]);
+      ss[threadIdx.x] = a;
+      sc[threadIdx.x] = ca;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *dst = make_double2(ss[0], sc[0]);
+}
+
+// ---------------------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------------------
+static hipError_t record(hipEvent_t ev, hipStream_t st) { return ev ? hipEventRecord(ev, st) : hipSuccess; }
+
+template <int N>
+static hipError_t launch_step_n(const StepLaunch& sl, hipStream_t st) {
+  using CF = Cfg<N>;
+  constexpr int TC = CF::TC;
+  hipError_t e;
+  if ((e = record(sl.ev[0], st)) != hipSuccess) return e;
+  const dim3 g_row((sl.max_c * N + CF::ROWS - 1) / CF::ROWS, 1, sl.count);
+  k_row_fwd<N><<<g_row, CF::ROW_THREADS, 0, st>>>(sl.descs, sl.parity, sl.tw);
+  if ((e = record(sl.ev[1], st)) != hipSuccess) return e;
+  const dim3 g_col((N / 2) / TC, sl.max_c, sl.count);
+  k_col_fused<N, TC><<<g_col, TC * CF::T_COL, 0, st>>>(sl.descs, sl.tw);
+  if ((e = record(sl.ev[2], st)) != hipSuccess) return e;
+  const dim3 g_inv(N / CF::ROWS, 1, sl.count);
+  if (sl.write_u)
+    k_row_inv_growth<N, true><<<g_inv, CF::ROW_THREADS, 0, st>>>(sl.descs, sl.tw);
+  else
+    k_row_inv_growth<N, false><<<g_inv, CF::ROW_THREADS, 0, st>>>(sl.descs, sl.tw);
+  if ((e = record(sl.ev[3], st)) != hipSuccess) return e;
+  if (!sl.skip_diffusion) {
+    const int tiles_x = (N + kDiffTW - 1) / kDiffTW, tiles_y = (N + kDiffTH - 1) / kDiffTH;
+    const dim3 g_dif(tiles_x, tiles_y * sl.max_c, sl.count);
+    k_diffusion<<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.parity, N);
+  }
+  if ((e = record(sl.ev[4], st)) != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+hipError_t launch_step(int N, const StepLaunch& sl, hipStream_t st) {
+  switch (N) {
+    case 16: return launch_step_n<16>(sl, st);
+    case 32: return launch_step_n<32>(sl, st);
+    case 64: return launch_step_n<64>(sl, st);
+    case 128: return launch_step_n<128>(sl, st);
+    case 256: return launch_step_n<256>(sl, st);
+    case 512: return launch_step_n<512>(sl, st);
+    case 1024: return launch_step_n<1024>(sl, st);
+    case 2048: return launch_step_n<2048>(sl, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_mass(const CreatureDev* descs, int count, int parity, int max_c, double2* partial, int nblk,
+                       hipStream_t st) {
+  k_mass<<<dim3(nblk, max_c, count), kMassThreads, 0, st>>>(descs, parity, partial, nblk);
+  return hipGetLastError();
+}
+
+}  // namespace cn

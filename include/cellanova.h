@@ -1,0 +1,177 @@
+/* cellanova.h - C ABI of the MI355X DiffusionLenia step engine.
+ *
+ * Drop-in boundary for the reference orchestrator `GpuMaceLenia` (ml-rs/orchestrator/mod.rs)
+ * and its loader (ml-rs/config.rs).  Each entry point cites the reference interface it
+ * replaces.  Conventions:
+ *   - every call returns cn_status (CN_OK == 0); nothing aborts across the boundary and the
+ *     last error text of the calling thread is available from cn_last_error();
+ *   - arguments are validated on the host before any HIP call is made;
+ *   - host arrays are caller-owned and their lengths are passed explicitly; device memory is
+ *     owned by the engine; one HIP stream per engine;
+ *   - a handle is used by one thread at a time (many handles may coexist);
+ *   - cn_iterate / cn_run_steps are asynchronous (like `iterate`, mod.rs:385); downloads,
+ *     cn_sync, cn_timed_iterate and cn_mass_fp64 synchronise.
+ */
+#ifndef CELLANOVA_H
+#define CELLANOVA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cn_status {
+  CN_OK = 0,
+  CN_ERR_INVALID_ARG = 1,   /* bad pointer, size, index, parameter value */
+  CN_ERR_UNSUPPORTED = 2,   /* valid but unsupported configuration (e.g. N not a power of two) */
+  CN_ERR_HIP = 3,           /* a HIP runtime call failed */
+  CN_ERR_IO = 4,            /* file missing / unreadable / truncated */
+  CN_ERR_PARSE = 5,         /* malformed JSON */
+  CN_ERR_OUT_OF_MEMORY = 6, /* device or host allocation failed */
+  CN_ERR_STATE = 7          /* call not valid in the current engine state */
+} cn_status;
+
+typedef struct cn_engine cn_engine;
+
+/* Engine construction parameters.  Replaces the argument list of GpuMaceLenia::new
+ * (ml-rs/orchestrator/mod.rs:79-90): shape -> n (square grid, power of two, 16..2048),
+ * num_channels, num_kernels, c0 -> src (input channel of kernel k), c1 -> dst (output
+ * channel of kernel k), mu, sigma, weights, temp.  C <= 16 (compute_1024.wgsl:307), K <= 256. */
+typedef struct cn_engine_desc {
+  uint32_t n;
+  uint32_t num_channels;
+  uint32_t num_kernels;
+  const uint32_t* src;   /* [K] */
+  const uint32_t* dst;   /* [K] */
+  const float* mu;       /* [K] */
+  const float* sigma;    /* [K], > 0 */
+  const float* weights;  /* [K] */
+  float temp;
+  int device;            /* HIP device ordinal */
+} cn_engine_desc;
+
+/* GpuMaceLenia::new (mod.rs:79-198).  All state starts at zero. */
+cn_status cn_engine_create(const cn_engine_desc* desc, cn_engine** out);
+/* Drop of GpuMaceLenia (buffers released). */
+cn_status cn_engine_destroy(cn_engine* e);
+
+/* GpuMaceLenia::set_kernel (mod.rs:207-214): full N*N complex64 spectrum of kernel k as
+ * interleaved (re, im) f32, row-major, unnormalised FFT of the origin-centred spatial kernel.
+ * n_complex must equal N*N (the reference asserts this, mod.rs:208). */
+cn_status cn_set_kernel(cn_engine* e, uint32_t k, const float* interleaved, size_t n_complex);
+
+/* GpuMaceLenia::upload_channel (mod.rs:217-226): f64 values cast to f32; n must equal N*N. */
+cn_status cn_upload_channel_f64(cn_engine* e, uint32_t channel, const double* data, size_t n);
+/* Same with f32 input (no cast). */
+cn_status cn_upload_channel_f32(cn_engine* e, uint32_t channel, const float* data, size_t n);
+
+/* GpuMaceLenia::iterate (mod.rs:348-386): one step, asynchronous. */
+cn_status cn_iterate(cn_engine* e);
+/* GpuMaceLenia::run_steps (mod.rs:229-233): n steps, asynchronous (HIP-graph replay). */
+cn_status cn_run_steps(cn_engine* e, uint64_t n);
+/* Wait for all queued work of the engine (device.poll(Wait) in the reference). */
+cn_status cn_sync(cn_engine* e);
+
+/* GpuMaceLenia::timed_iterate (mod.rs:469-540): one step with per-phase device times in ms:
+ * ms[0] convolution (row + fused column passes), ms[1] growth (inverse row pass fused with
+ * the growth/affinity epilogue), ms[2] diffusion, ms[3] copy (0: ping-pong pointer swap). */
+cn_status cn_timed_iterate(cn_engine* e, double ms[4]);
+
+/* New (extends timed_iterate): run `steps` directly launched steps with HIP events around every
+ * kernel on the engine's stream and return the average device time per launch in ms:
+ * kernel_ms[0] row forward (R2C), [1] fused column pass, [2] inverse row + growth, [3] diffusion.
+ * n must be 4.  Synchronises. */
+cn_status cn_profile_kernels(cn_engine* e, uint64_t steps, double* kernel_ms, size_t n);
+
+/* GpuMaceLenia::download_all_channels (mod.rs:236-263): C*N*N f32, channel-major. */
+cn_status cn_download_channels(cn_engine* e, float* out, size_t n);
+/* GpuMaceLenia::download_conv_buffer (mod.rs:267-299): K*N*N f32, the real part of the
+ * UNNORMALISED inverse transform (N^2 times the convolution) computed from the state the
+ * most recent step started from; zeros before the first step. */
+cn_status cn_download_conv_real(cn_engine* e, float* out, size_t n);
+
+/* New: per-channel total mass of the current state, FP64 Neumaier-compensated sums on the
+ * device (out[c], c < C).  Synchronises. */
+cn_status cn_mass_fp64(cn_engine* e, double* out, size_t n_channels);
+
+/* New: engine facts. */
+typedef struct cn_engine_info {
+  uint32_t n, num_channels, num_kernels;
+  uint64_t steps_done;
+  uint64_t device_bytes;   /* device memory held by the engine */
+} cn_engine_info;
+cn_status cn_engine_get_info(const cn_engine* e, cn_engine_info* out);
+
+/* Text of the calling thread's last error ("" if none).  The pointer stays valid until the
+ * next cn_* call on the same thread. */
+const char* cn_last_error(void);
+
+/* ------------------------------------------------------------------------------------ */
+/* Loader (ml-rs/config.rs)                                                              */
+/* ------------------------------------------------------------------------------------ */
+
+/* SimConfig (config.rs:14-25) after load_config's c0 <-> c1 swap (config.rs:40). */
+typedef struct cn_config {
+  uint32_t seed_size;
+  uint32_t num_channels;
+  uint32_t num_kernels;
+  double* seed_channels;   /* [num_channels][seed_size*seed_size] */
+  size_t seed_len;         /* elements per channel actually present (must be seed_size^2) */
+  uint32_t* c0;            /* [K] input channel (= JSON c1) */
+  uint32_t* c1;            /* [K] output channel (= JSON c0) */
+  float* growth_mu;        /* [K] */
+  float* growth_sigma;     /* [K] */
+  float* growth_weights;   /* [K] */
+} cn_config;
+
+/* load_config (config.rs:32-47): parse the JSON file and swap c0/c1. */
+cn_status cn_load_config(const char* path, cn_config** out);
+void cn_free_config(cn_config* cfg);
+
+/* load_kernels (config.rs:56-125): read K spectra of S*S complex64 (S inferred from the
+ * file size) and, if S != target, zero-pad them spectrally to target*target (fftshift,
+ * centre pad, ifftshift).  out receives K*target*target interleaved (re, im) pairs;
+ * n_floats must equal 2*K*target*target.  A file whose size is not K*S*S*8 bytes is
+ * rejected (the reference would silently mis-size it, config.rs:62-65), as is target < S. */
+cn_status cn_load_kernels(const char* path, uint32_t num_kernels, uint32_t target, float* out,
+                          size_t n_floats);
+
+/* ------------------------------------------------------------------------------------ */
+/* Synthetic creatures (no reference counterpart: the benchmark distributions)           */
+/* ------------------------------------------------------------------------------------ */
+
+/* Lenia ring-kernel creature.  Kernel k: radius R, B_k ~ U{1,2,3} rings with peaks
+ * beta ~ U[0.3,1], core exp(4 - 1/(f(1-f))), normalised to sum 1 in FP64, origin at (0,0),
+ * FP64 FFT rounded to complex64.  mu ~ U[0.05,0.15], sigma ~ U[0.03,0.06], weights
+ * Dirichlet(1) over the kernels sharing a destination.  Seed: U[0,1) on the central
+ * patch x patch square of every channel, zero elsewhere.  All draws come from a
+ * counter-based generator keyed by (seed, creature, stream, index). */
+typedef struct cn_creature_params {
+  uint32_t n, num_channels, num_kernels;
+  const uint32_t* src;      /* [K] */
+  const uint32_t* dst;      /* [K] */
+  const uint32_t* radius;   /* [K] kernel radii R (cells) */
+  uint32_t patch;           /* seed patch side (clamped to n) */
+  uint64_t seed;
+  uint64_t creature;
+} cn_creature_params;
+
+/* Outputs (any may be NULL): kernels_fft [K][n*n] interleaved complex64 (2*K*n*n floats),
+ * spatial [K][n*n] f64 kernels, state [C][n*n] f32, mu/sigma/weights [K]. */
+cn_status cn_generate_creature(const cn_creature_params* p, float* kernels_fft, double* spatial,
+                               float* state, float* mu, float* sigma, float* weights);
+
+/* ------------------------------------------------------------------------------------ */
+/* Test hooks                                                                            */
+/* ------------------------------------------------------------------------------------ */
+
+/* Make the next `count` HIP calls issued through the engine's checked wrapper report failure
+ * without calling HIP (exercises error paths on machines without a GPU). */
+void cn_debug_fail_next_hip(int count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CELLANOVA_H */
