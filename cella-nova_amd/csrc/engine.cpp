@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 #include "common.h"
@@ -143,6 +144,7 @@ struct cn_engine {
   std::vector<void*> allocs;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int col_tc = 0;  // fused column pass tile width (0: built-in default for N)
 };
 
 namespace {
@@ -175,6 +177,7 @@ StepLaunch base_launch(cn_engine* e, int parity) {
   sl.parity = parity;
   sl.max_c = (int)e->C;
   sl.tw = e->tw;
+  sl.col_tc = e->col_tc;
   return sl;
 }
 
@@ -221,6 +224,7 @@ cn_status cn_engine_create(const cn_engine_desc* desc, cn_engine** out) {
   cn_engine* e = new cn_engine();
   e->device = desc->device;
   e->N = desc->n;
+  if (const char* tc = std::getenv("CN_COL_TC")) e->col_tc = std::atoi(tc);
   e->C = desc->num_channels;
   e->K = desc->num_kernels;
   auto fail = [&](cn_status s) {

@@ -13,6 +13,7 @@ struct StepLaunch {
   const float2* tw = nullptr;          // kTwN-entry twiddle table
   bool write_u = false;                // also write the real convolution results to desc.u
   bool skip_diffusion = false;         // stop after the affinity (conv readback)
+  int col_tc = 0;                      // column tile width of the fused column pass (0: default)
   // Optional per-kernel timing events: ev[0] before the row pass, ev[1] after it, ev[2] after
   // the column pass, ev[3] after the inverse-row + growth pass, ev[4] after the diffusion.
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};

@@ -96,7 +96,13 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
   if (c >= d.C) return;
   const int kb = d.src_off[c], ke = d.src_off[c + 1];
   if (kb == ke) return;
-  const int x0 = blockIdx.x * TC;
+  // Workgroups are dealt round-robin to the 8 XCDs (separate L2s).  Give each XCD a
+  // contiguous run of tiles so that neighbouring tiles, which share 128-byte lines of every
+  // row, are fetched through the same L2.
+  constexpr int NTILES = (N / 2) / TC;
+  const int bx = blockIdx.x;
+  const int tile = (NTILES % 8 == 0) ? (bx % 8) * (NTILES / 8) + bx / 8 : bx;
+  const int x0 = tile * TC;
   const int lane = threadIdx.x % TC;
   int t = threadIdx.x / TC;
   const int x = x0 + lane;
@@ -178,12 +184,13 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
   constexpr int L = CF::L_ROW, E = CF::E_ROW, T = CF::T_ROW, ROWS = CF::ROWS;
   const CreatureDev& d = descs[blockIdx.z];
   if (d.N != N) return;
+  const int c = blockIdx.y;  // destination channel
+  if (c >= d.C) return;
   const int lane = threadIdx.x / T, t0 = threadIdx.x % T;
   const int y = blockIdx.x * ROWS + lane;  // ROWS divides N
   __shared__ float2 lds[LdsMap<L, ROWS, false>::size];
   const float temp = d.temp;
-
-  for (int c = 0; c < d.C; ++c) {
+  {
     float2 acc[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) acc[i] = make_float2(0.f, 0.f);
@@ -235,8 +242,22 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
 
 // ---------------------------------------------------------------------------------------
 // Diffusion: Z = box3(A) (periodic), new = A * box3(state / Z).  Halo-2 tile in LDS.
+//
+// Mass conservation in FP32.  Exactly, sum_p new_p = sum_n (s_n / Z_n) * sum_{p in N(n)} A_p
+// = sum_n s_n.  Plain FP32 breaks the identity by a systematic ~5e-8 per step (the reference's
+// f32 CPU step drifts the same way): Z_n is not exactly the sum of the A_p that later receive
+// q_n, q_n*Z_n != s_n, and in smooth regions these roundings repeat with the same sign.  Here
+// Z and the gather sums use error-free TwoSum chains (hi + lo), each site keeps the exact
+// residual r_n = s_n - q_n*(Z_hi + Z_lo) of its division (FMA), and adds it back to its own
+// output.  What is left is the final rounding of each output (random sign).
 // ---------------------------------------------------------------------------------------
 constexpr int kDiffTW = 64, kDiffTH = 16, kDiffThreads = 256;
+
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+  s = a + b;
+  const float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
 
 __global__ void __launch_bounds__(kDiffThreads)
 k_diffusion(const CreatureDev* __restrict__ descs, int parity, int N) {
@@ -256,6 +277,7 @@ k_diffusion(const CreatureDev* __restrict__ descs, int parity, int N) {
 
   __shared__ float sA[TH + 4][TW + 5];
   __shared__ float sQ[TH + 2][TW + 3];
+  __shared__ float sR[TH + 2][TW + 3];
   const int tid = threadIdx.x;
   for (int i = tid; i < (TH + 4) * (TW + 4); i += kDiffThreads) {
     const int r = i / (TW + 4), cc = i % (TW + 4);
@@ -265,24 +287,35 @@ k_diffusion(const CreatureDev* __restrict__ descs, int parity, int N) {
   __syncthreads();
   for (int i = tid; i < (TH + 2) * (TW + 2); i += kDiffThreads) {
     const int r = i / (TW + 2), cc = i % (TW + 2);
-    float z = 0.f;
+    float zh = sA[r][cc], zl = 0.f;
 #pragma unroll
-    for (int dr = 0; dr < 3; ++dr)
-#pragma unroll
-      for (int dc = 0; dc < 3; ++dc) z += sA[r + dr][cc + dc];
+    for (int j = 1; j < 9; ++j) {
+      float e;
+      two_sum(zh, sA[r + j / 3][cc + j % 3], zh, e);
+      zl += e;
+    }
     const int gy = (y0 - 1 + r) & mask, gx = (x0 - 1 + cc) & mask;
-    sQ[r][cc] = s[(size_t)gy * N + gx] / z;
+    const float sv = s[(size_t)gy * N + gx];
+    const float q = sv / zh;
+    sQ[r][cc] = q;
+    sR[r][cc] = fmaf(-q, zl, fmaf(-q, zh, sv));  // s - q*(zh + zl)
   }
   __syncthreads();
   for (int i = tid; i < TH * TW; i += kDiffThreads) {
     const int r = i / TW, cc = i % TW;
     const int gy = y0 + r, gx = x0 + cc;
-    float acc = 0.f;
+    float sh = sQ[r][cc], sl = 0.f;
 #pragma unroll
-    for (int dr = 0; dr < 3; ++dr)
-#pragma unroll
-      for (int dc = 0; dc < 3; ++dc) acc += sQ[r + dr][cc + dc];
-    if (gy < N && gx < N) o[(size_t)gy * N + gx] = sA[r + 2][cc + 2] * acc;
+    for (int j = 1; j < 9; ++j) {
+      float e;
+      two_sum(sh, sQ[r + j / 3][cc + j % 3], sh, e);
+      sl += e;
+    }
+    const float a = sA[r + 2][cc + 2];
+    const float t = a * sh;
+    const float err = fmaf(a, sh, -t);
+    const float v = t + (err + fmaf(a, sl, sR[r + 1][cc + 1]));
+    if (gy < N && gx < N) o[(size_t)gy * N + gx] = v;
   }
 }
 
@@ -331,19 +364,35 @@ k_mass(const CreatureDev* __restrict__ descs, int parity, double2* __restrict__ 
 // ---------------------------------------------------------------------------------------
 static hipError_t record(hipEvent_t ev, hipStream_t st) { return ev ? hipEventRecord(ev, st) : hipSuccess; }
 
+template <int N, int TC>
+static hipError_t launch_col(const StepLaunch& sl, hipStream_t st) {
+  if constexpr (TC > N / 2 || (size_t)N * TC * 8 + (size_t)N * 8 > 160 * 1024) {
+    return hipErrorInvalidValue;
+  } else {
+    const dim3 g_col((N / 2) / TC, sl.max_c, sl.count);
+    k_col_fused<N, TC><<<g_col, TC * Cfg<N>::T_COL, 0, st>>>(sl.descs, sl.tw);
+    return hipSuccess;
+  }
+}
+
 template <int N>
 static hipError_t launch_step_n(const StepLaunch& sl, hipStream_t st) {
   using CF = Cfg<N>;
-  constexpr int TC = CF::TC;
   hipError_t e;
   if ((e = record(sl.ev[0], st)) != hipSuccess) return e;
   const dim3 g_row((sl.max_c * N + CF::ROWS - 1) / CF::ROWS, 1, sl.count);
   k_row_fwd<N><<<g_row, CF::ROW_THREADS, 0, st>>>(sl.descs, sl.parity, sl.tw);
   if ((e = record(sl.ev[1], st)) != hipSuccess) return e;
-  const dim3 g_col((N / 2) / TC, sl.max_c, sl.count);
-  k_col_fused<N, TC><<<g_col, TC * CF::T_COL, 0, st>>>(sl.descs, sl.tw);
+  switch (sl.col_tc ? sl.col_tc : CF::TC) {
+    case 2: e = launch_col<N, 2>(sl, st); break;
+    case 4: e = launch_col<N, 4>(sl, st); break;
+    case 8: e = launch_col<N, 8>(sl, st); break;
+    case 16: e = launch_col<N, 16>(sl, st); break;
+    default: e = hipErrorInvalidValue;
+  }
+  if (e != hipSuccess) return e;
   if ((e = record(sl.ev[2], st)) != hipSuccess) return e;
-  const dim3 g_inv(N / CF::ROWS, 1, sl.count);
+  const dim3 g_inv(N / CF::ROWS, sl.max_c, sl.count);
   if (sl.write_u)
     k_row_inv_growth<N, true><<<g_inv, CF::ROW_THREADS, 0, st>>>(sl.descs, sl.tw);
   else
