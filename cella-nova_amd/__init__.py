@@ -3,7 +3,7 @@ orchestrator and its loader, over the C ABI in include/cellanova.h (libcellanova
 from ._lib import CellaNovaError, LIB_PATH, lib
 from .config import SimConfig, load_config, load_kernels
 from .creatures import baseline_creature, channel_maps, generate_creature, make_engine
-from .engine import GpuMaceLenia
+from .engine import CreatureBatch, GpuMaceLenia
 
-__all__ = ["CellaNovaError", "GpuMaceLenia", "LIB_PATH", "SimConfig", "baseline_creature", "channel_maps",
+__all__ = ["CellaNovaError", "CreatureBatch", "GpuMaceLenia", "LIB_PATH", "SimConfig", "baseline_creature", "channel_maps",
            "generate_creature", "lib", "load_config", "load_kernels", "make_engine"]

@@ -81,6 +81,24 @@ SIGNATURES = {
     "cn_generate_creature": (c_int, [POINTER(CreatureParams), c_void_p, c_void_p, c_void_p, c_void_p,
                                      c_void_p, c_void_p]),
     "cn_debug_fail_next_hip": (None, [c_int]),
+    # batches
+    "cn_batch_create": (c_int, [POINTER(EngineDesc), c_uint32, c_int, POINTER(c_void_p)]),
+    "cn_creature_set_kernel": (c_int, [c_void_p, c_uint32, c_uint32, c_void_p, c_size_t]),
+    "cn_creature_upload_channel_f32": (c_int, [c_void_p, c_uint32, c_uint32, c_void_p, c_size_t]),
+    "cn_creature_download_channels": (c_int, [c_void_p, c_uint32, c_void_p, c_size_t]),
+    "cn_creature_mass_fp64": (c_int, [c_void_p, c_uint32, c_void_p, c_size_t]),
+    "cn_creature_set_growth": (c_int, [c_void_p, c_uint32, c_void_p, c_void_p, c_void_p, c_size_t]),
+    "cn_creature_state_from_device": (c_int, [c_void_p, c_uint32, c_void_p, c_size_t]),
+    "cn_creature_state_to_device": (c_int, [c_void_p, c_uint32, c_void_p, c_size_t]),
+    # rollouts
+    "cn_creature_set_target": (c_int, [c_void_p, c_uint32, c_void_p, c_size_t]),
+    "cn_rollout_grad": (c_int, [c_void_p, c_uint32, c_uint32]),
+    "cn_rollout_forward": (c_int, [c_void_p, c_uint32, c_uint32]),
+    "cn_creature_rollout_final": (c_int, [c_void_p, c_uint32, c_void_p, c_size_t, c_int]),
+    "cn_rollout_backward": (c_int, [c_void_p, POINTER(c_void_p), c_uint32]),
+    "cn_creature_get_loss": (c_int, [c_void_p, c_uint32, POINTER(c_double)]),
+    "cn_creature_get_grads": (c_int, [c_void_p, c_uint32, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p,
+                                      c_size_t, c_int]),
 }
 
 _lib = None

@@ -1,0 +1,50 @@
+"""torch.autograd bridge for the checkpointed DiffusionLenia rollout (C ABI cn_rollout_forward /
+cn_rollout_backward).  Sits beside the reference's PyTorch model (train/diff_lenia_org.py): where
+that model is differentiated by autograd through torch.fft and F.unfold, this Function runs the
+hand-written forward and adjoint HIP kernels and keeps only a checkpoint every `ckpt_every`
+steps, recomputing each segment during the backward pass.
+
+One engine holds one outstanding rollout: call backward before the next forward on it.
+"""
+import torch
+
+from .engine import GpuMaceLenia
+
+
+class DiffusionLeniaRollout(torch.autograd.Function):
+    """x_T = rollout(x_0; mu, sigma, weights) with gradients for x_0, mu, sigma and weights."""
+
+    @staticmethod
+    def forward(ctx, state0, mu, sigma, weights, engine, steps, ckpt_every):
+        if not isinstance(engine, GpuMaceLenia):
+            raise TypeError("engine must be a GpuMaceLenia")
+        if state0.dtype != torch.float32 or not state0.is_cuda:
+            raise TypeError("state0 must be a float32 device tensor")
+        engine.set_growth(mu.detach().cpu().numpy(), sigma.detach().cpu().numpy(),
+                          weights.detach().cpu().numpy())
+        s0 = state0.detach().contiguous()
+        torch.cuda.current_stream().synchronize()
+        engine.creature_state_from_device(0, s0)
+        engine.rollout_forward(steps, ckpt_every)
+        out = torch.empty_like(s0)
+        engine.creature_rollout_final(0, out)
+        ctx.engine = engine
+        ctx.K = mu.numel()
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        eng = ctx.engine
+        g = grad_out.detach().to(torch.float32).contiguous()
+        torch.cuda.current_stream().synchronize()
+        eng.rollout_backward([g])
+        dev = g.device
+        gs = torch.empty_like(g)
+        gm, gsg, gw = (torch.empty(ctx.K, dtype=torch.float32, device=dev) for _ in range(3))
+        eng.creature_grads(0, gs, gm, gsg, gw, device=True)
+        return gs, gm, gsg, gw, None, None, None
+
+
+def rollout(engine, state0, mu, sigma, weights, steps, ckpt_every=8):
+    """Differentiable T-step rollout through the HIP engine (see DiffusionLeniaRollout)."""
+    return DiffusionLeniaRollout.apply(state0, mu, sigma, weights, engine, int(steps), int(ckpt_every))

@@ -1,0 +1,397 @@
+// Adjoint (reverse-mode) kernels of one DiffusionLenia step.  No counterpart in the reference,
+// which has no differentiable GPU path; the PyTorch model (train/diff_lenia_org.py:27-53) is
+// differentiated by autograd instead.  Forward step, per channel c and kernel k:
+//   u_k = s_{src k} (*) K_k                      (circular convolution)
+//   G_k = 2 exp(-(u_k - mu_k)^2 / (2 sigma_k^2)) - 1
+//   A_c = exp(temp * sum_{k: dst k = c} w_k G_k)
+//   Z_c = box3(A_c),  q_c = s_c / Z_c,  s'_c = A_c * box3(q_c)
+// Given g = dL/ds', one adjoint step computes (with the forward intermediates recomputed):
+//   T1 = box3(g A),  dL/ds (diffusion part) = T1 / Z,  dZ = -T1 q / Z
+//   dL/dA = g box3(q) + box3(dZ),  gaff = dL/daff = temp A dL/dA
+//   gu_k = w_k gaff_{dst k} (G_k + 1) (-(u_k - mu_k) / sigma_k^2)
+//   dL/dmu_k = -sum gu_k,  dL/dsigma_k = -sum gu_k (u_k - mu_k) / sigma_k,  dL/dw_k = sum G_k gaff
+//   dL/ds_c (convolution part) = sum_{k: src k = c} gu_k (correlated with) K_k
+//                              = IFFT(sum_k FFT(gu_k) conj(K^_k))   (one inverse per source)
+// Launches: k_diffusion_adj, k_growth_adj_row (growth adjoint + row R2C of gu_k),
+// k_col_adj (forward column FFT of every gu_k, product with the conjugate spectrum, summed per
+// source, one inverse column FFT), k_row_inv_add (row C2R + diffusion part), k_param_reduce.
+#include <hip/hip_runtime.h>
+#include "cfg.h"
+#include "fft.h"
+#include "launch.h"
+#include "realfft.h"
+#include "types.h"
+
+namespace cn {
+
+constexpr int kAdjTW = 64, kAdjTH = 16, kAdjThreads = 256;
+
+// ---------------------------------------------------------------------------------------
+// Diffusion adjoint: inputs A (recomputed), state slot `sin`, gradient slot `gin`;
+// outputs dsd = T1 / Z and gaff.  Halo-2 tiles of A and g in LDS.
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kAdjThreads)
+k_diffusion_adj(const CreatureDev* __restrict__ descs, int sin, int gin, int N) {
+  constexpr int TW = kAdjTW, TH = kAdjTH;
+  const CreatureDev& d = descs[blockIdx.z];
+  if (d.N != N) return;
+  const int tiles_y = (N + TH - 1) / TH;
+  const int c = blockIdx.y / tiles_y;
+  if (c >= d.C) return;
+  const int y0 = (blockIdx.y % tiles_y) * TH;
+  const int x0 = blockIdx.x * TW;
+  const int mask = N - 1;
+  const size_t plane = (size_t)N * N;
+  const float* A = d.A + c * plane;
+  const float* s = slot_ptr(d, sin) + c * plane;
+  const float* g = d.grad + ((size_t)gin * d.C + c) * plane;
+  float* dsd = d.dsd + c * plane;
+  float* gaff = d.gaff + c * plane;
+  const float temp = d.temp;
+
+  __shared__ float sA[TH + 4][TW + 5];
+  __shared__ float sG[TH + 4][TW + 5];
+  __shared__ float sQ[TH + 2][TW + 3];
+  __shared__ float sD[TH + 2][TW + 3];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < (TH + 4) * (TW + 4); i += kAdjThreads) {
+    const int r = i / (TW + 4), cc = i % (TW + 4);
+    const size_t o = (size_t)((y0 - 2 + r) & mask) * N + ((x0 - 2 + cc) & mask);
+    sA[r][cc] = A[o];
+    sG[r][cc] = g[o];
+  }
+  __syncthreads();
+  for (int i = tid; i < (TH + 2) * (TW + 2); i += kAdjThreads) {
+    const int r = i / (TW + 2), cc = i % (TW + 2);
+    float z = 0.f, t1 = 0.f;
+#pragma unroll
+    for (int dr = 0; dr < 3; ++dr)
+#pragma unroll
+      for (int dc = 0; dc < 3; ++dc) {
+        const float a = sA[r + dr][cc + dc];
+        z += a;
+        t1 += a * sG[r + dr][cc + dc];
+      }
+    const int gy = (y0 - 1 + r) & mask, gx = (x0 - 1 + cc) & mask;
+    const float q = s[(size_t)gy * N + gx] / z;
+    sQ[r][cc] = q;
+    sD[r][cc] = -t1 * q / z;
+    if (r >= 1 && r <= TH && cc >= 1 && cc <= TW && y0 - 1 + r < N && x0 - 1 + cc < N)
+      dsd[(size_t)gy * N + gx] = t1 / z;
+  }
+  __syncthreads();
+  for (int i = tid; i < TH * TW; i += kAdjThreads) {
+    const int r = i / TW, cc = i % TW;
+    const int gy = y0 + r, gx = x0 + cc;
+    float b = 0.f, dz = 0.f;
+#pragma unroll
+    for (int dr = 0; dr < 3; ++dr)
+#pragma unroll
+      for (int dc = 0; dc < 3; ++dc) {
+        b += sQ[r + dr][cc + dc];
+        dz += sD[r + dr][cc + dc];
+      }
+    const float a = sA[r + 2][cc + 2];
+    const float ga = sG[r + 2][cc + 2] * b + dz;
+    if (gy < N && gx < N) gaff[(size_t)gy * N + gx] = ga * temp * a;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Growth adjoint fused with the row R2C transform of gu_k; per-block sums for the
+// parameter gradients.  Grid (N/ROWS, K, creatures).
+// ---------------------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(Cfg<N>::ROW_THREADS)
+k_growth_adj_row(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) {
+  using CF = Cfg<N>;
+  constexpr int L = CF::L_ROW, E = CF::E_ROW, T = CF::T_ROW, ROWS = CF::ROWS, NT = CF::ROW_THREADS;
+  const CreatureDev& d = descs[blockIdx.z];
+  if (d.N != N) return;
+  const int k = blockIdx.y;
+  if (k >= d.K) return;
+  const int lane = threadIdx.x / T, t = threadIdx.x % T;
+  const int y = blockIdx.x * ROWS + lane;
+  const float mu = d.mu[k], is2 = d.inv_s2[k], wk = d.w[k], sg = d.sigma[k];
+  const int c = d.dst_of[k];
+  const float2* u = reinterpret_cast<const float2*>(d.u + ((size_t)k * N + y) * N);
+  const float2* ga = reinterpret_cast<const float2*>(d.gaff + ((size_t)c * N + y) * N);
+  float2 v[E];
+  float smu = 0.f, ssg = 0.f, sw = 0.f;
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int n = in_pos<L, E, false>(t, i);
+    const float2 uu = u[n], gg = ga[n];
+    const float d0 = uu.x - mu, d1 = uu.y - mu;
+    const float e0 = 2.0f * expf(-0.5f * d0 * d0 * is2), e1 = 2.0f * expf(-0.5f * d1 * d1 * is2);  // G + 1
+    const float gu0 = wk * gg.x * e0 * (-d0 * is2), gu1 = wk * gg.y * e1 * (-d1 * is2);
+    v[i] = make_float2(gu0, gu1);
+    smu -= gu0 + gu1;
+    ssg -= (gu0 * d0 + gu1 * d1) / sg;
+    sw += (e0 - 1.0f) * gg.x + (e1 - 1.0f) * gg.y;
+  }
+  __shared__ float2 lds[LdsMap<L, ROWS, false>::size];
+  __shared__ float red[3][NT];
+  red[0][threadIdx.x] = smu;
+  red[1][threadIdx.x] = ssg;
+  red[2][threadIdx.x] = sw;
+  fft_run<L, E, ROWS, false, false, kTwN>(v, lds, t, lane, tw);  // barriers inside cover red[]
+  __syncthreads();
+  for (int h = NT / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) red[j][threadIdx.x] += red[j][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) d.partial[((size_t)blockIdx.x * d.K + k) * 3 + threadIdx.x] = red[threadIdx.x][0];
+  r2c_finish<N, E, ROWS>(v, lds, t, lane, tw, reinterpret_cast<float2*>(d.specg) + ((size_t)k * N + y) * L, true);
+}
+
+// ---------------------------------------------------------------------------------------
+// Column adjoint: for one source channel and column tile, sum over the kernels reading that
+// source of FFT_col(rowspec(gu_k)) * conj(K^_k), then one inverse column FFT into spec[c].
+// ---------------------------------------------------------------------------------------
+template <int N, int TC>
+__global__ void __launch_bounds__(TC * Cfg<N>::T_COL) __attribute__((amdgpu_waves_per_eu(4)))
+k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) {
+  using CF = Cfg<N>;
+  constexpr int L = N, E = CF::E_COL, HALF = N / 2;
+  const CreatureDev& d = descs[blockIdx.z];
+  if (d.N != N) return;
+  const int c = blockIdx.y;
+  if (c >= d.C) return;
+  const int kb = d.src_off[c], ke = d.src_off[c + 1];
+  const int x0 = xcd_tile<(N / 2) / TC>(blockIdx.x) * TC;
+  const int lane = threadIdx.x % TC;
+  int t = threadIdx.x / TC;
+  const int x = x0 + lane;
+  const bool edge = (x0 == 0);
+  __shared__ float2 lds[L * TC];
+  __shared__ float2 ldsB[N];
+
+  float2 acc[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) acc[i] = make_float2(0.f, 0.f);
+  for (int kk = kb; kk < ke; ++kk) {
+    const int k = d.src_k[kk];
+    t = launder(t);
+    const float2* sp = reinterpret_cast<const float2*>(d.specg) + (size_t)k * N * HALF;
+    const float2* kh = reinterpret_cast<const float2*>(d.khalf) + (size_t)k * N * HALF;
+    float2 v[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) v[i] = sp[(size_t)in_pos<L, E, false>(t, i) * HALF + x];
+    fft_run<L, E, TC, true, false, kTwN>(v, lds, t, lane, tw);
+    if (edge) {
+      // Column 0 packs (DC column) + i (Nyquist column): split, multiply each by its
+      // conjugate kernel column, re-pack.
+      __syncthreads();
+      if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) ldsB[out_pos<L, E, false>(t, i)] = v[i];
+      }
+      __syncthreads();
+      if (lane == 0) {
+        const float2* kn = reinterpret_cast<const float2*>(d.knyq) + (size_t)k * N;
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+          const int y = out_pos<L, E, false>(t, i);
+          const float2 F = v[i];
+          const float2 G = ldsB[(N - y) & (N - 1)];
+          const float2 a = make_float2(0.5f * (F.x + G.x), 0.5f * (F.y - G.y));
+          const float2 dd = make_float2(F.x - G.x, F.y + G.y);
+          const float2 b = make_float2(0.5f * dd.y, -0.5f * dd.x);
+          const float2 pa = c_mulc(a, kh[(size_t)y * HALF]);
+          const float2 pb = c_mulc(b, kn[y]);
+          acc[i] = c_add(acc[i], make_float2(pa.x - pb.y, pa.y + pb.x));
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+          const int y = out_pos<L, E, false>(t, i);
+          acc[i] = c_add(acc[i], c_mulc(v[i], kh[(size_t)y * HALF + x]));
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const int y = out_pos<L, E, false>(t, i);
+        acc[i] = c_add(acc[i], c_mulc(v[i], kh[(size_t)y * HALF + x]));
+      }
+    }
+  }
+  fft_run<L, E, TC, true, true, kTwN>(acc, lds, t, lane, tw);
+  float2* out = reinterpret_cast<float2*>(d.spec) + (size_t)c * N * HALF;
+#pragma unroll
+  for (int i = 0; i < E; ++i) out[(size_t)out_pos<L, E, true>(t, i) * HALF + x] = acc[i];
+}
+
+// ---------------------------------------------------------------------------------------
+// Row C2R of spec[c] (the convolution part of dL/ds) plus the diffusion part -> grad slot.
+// ---------------------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(Cfg<N>::ROW_THREADS)
+k_row_inv_add(const CreatureDev* __restrict__ descs, int gout, const float2* __restrict__ tw) {
+  using CF = Cfg<N>;
+  constexpr int L = CF::L_ROW, E = CF::E_ROW, T = CF::T_ROW, ROWS = CF::ROWS;
+  const CreatureDev& d = descs[blockIdx.z];
+  if (d.N != N) return;
+  const int c = blockIdx.y;
+  if (c >= d.C) return;
+  const int lane = threadIdx.x / T, t = threadIdx.x % T;
+  const int y = blockIdx.x * ROWS + lane;
+  __shared__ float2 lds[LdsMap<L, ROWS, false>::size];
+  float2 v[E];
+  c2r_start<N, E>(reinterpret_cast<const float2*>(d.spec) + ((size_t)c * N + y) * L, t, tw, v);
+  fft_run<L, E, ROWS, false, true, kTwN>(v, lds, t, lane, tw);
+  const size_t row = ((size_t)c * N + y) * N;
+  const float2* dd = reinterpret_cast<const float2*>(d.dsd + row);
+  float2* o = reinterpret_cast<float2*>(d.grad + (size_t)gout * d.C * N * N + row);
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int n = out_pos<L, E, true>(t, i);
+    o[n] = c_add(dd[n], v[i]);
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Parameter gradients: sum the per-block partials (fixed order) and add them to the
+// accumulated gradients with Kahan compensation.  gparam = [mu | sigma | w] then the
+// compensation terms.
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_param_reduce(const CreatureDev* __restrict__ descs, int nblk_of_n_shift) {
+  const CreatureDev& d = descs[blockIdx.z];
+  const int K = d.K;
+  const int nblk = d.N >> nblk_of_n_shift;  // row blocks per plane
+  for (int idx = threadIdx.x; idx < 3 * K; idx += blockDim.x) {
+    const int k = idx / 3, j = idx % 3;
+    float s = 0.f, comp = 0.f;
+    for (int b = 0; b < nblk; ++b) {
+      const float yv = d.partial[((size_t)b * K + k) * 3 + j] - comp;
+      const float tt = s + yv;
+      comp = (tt - s) - yv;
+      s = tt;
+    }
+    float* acc = d.gparam + j * K + k;
+    float* cmp = d.gparam + 3 * K + j * K + k;
+    const float yv = s - *cmp;
+    const float tt = *acc + yv;
+    *cmp = (tt - *acc) - yv;
+    *acc = tt;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_zero_gparam(const CreatureDev* __restrict__ descs) {
+  const CreatureDev& d = descs[blockIdx.z];
+  for (int i = threadIdx.x; i < 6 * d.K; i += blockDim.x) d.gparam[i] = 0.f;
+}
+
+// ---------------------------------------------------------------------------------------
+// MSE loss against the target: g = 2 (x - target) / (C N^2) into grad slot gout, per-block
+// partial sums of (x - target)^2 into lossp.
+// ---------------------------------------------------------------------------------------
+constexpr int kLossThreads = 256, kLossBlocks = kLossPartials;
+
+__global__ void __launch_bounds__(kLossThreads)
+k_mse_grad(const CreatureDev* __restrict__ descs, int sfin, int gout) {
+  const CreatureDev& d = descs[blockIdx.z];
+  const size_t n = (size_t)d.C * d.N * d.N;
+  const float* x = slot_ptr(d, sfin);
+  float* g = d.grad + (size_t)gout * n;
+  const float scale = 2.0f / (float)n;
+  float acc = 0.f;
+  for (size_t i = (size_t)blockIdx.x * kLossThreads + threadIdx.x; i < n; i += (size_t)kLossBlocks * kLossThreads) {
+    const float r = x[i] - d.target[i];
+    g[i] = scale * r;
+    acc += r * r;
+  }
+  __shared__ float red[kLossThreads];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = kLossThreads / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) d.lossp[blockIdx.x] = red[0];
+}
+
+// ---------------------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------------------
+template <int N, int TC>
+static hipError_t launch_col_adj(const StepLaunch& sl, hipStream_t st) {
+  if constexpr (!col_tc_ok<N, TC>()) {
+    return hipErrorInvalidValue;
+  } else {
+    k_col_adj<N, TC><<<dim3((N / 2) / TC, sl.max_c, sl.count), TC * Cfg<N>::T_COL, 0, st>>>(sl.descs, sl.tw);
+    return hipSuccess;
+  }
+}
+
+// One adjoint step: recompute the forward intermediates of state slot sl.sin (u, A), then
+// grad slot gin -> gout and parameter-gradient accumulation.
+template <int N>
+static hipError_t launch_adjoint_n(const StepLaunch& sl, int gin, int gout, hipStream_t st) {
+  using CF = Cfg<N>;
+  StepLaunch fw = sl;
+  fw.write_u = true;
+  fw.skip_diffusion = true;
+  fw.ev[0] = fw.ev[1] = fw.ev[2] = fw.ev[3] = fw.ev[4] = nullptr;
+  hipError_t e = launch_step(N, fw, st);
+  if (e != hipSuccess) return e;
+  const int tiles_x = (N + kAdjTW - 1) / kAdjTW, tiles_y = (N + kAdjTH - 1) / kAdjTH;
+  k_diffusion_adj<<<dim3(tiles_x, tiles_y * sl.max_c, sl.count), kAdjThreads, 0, st>>>(sl.descs, sl.sin, gin, N);
+  k_growth_adj_row<N><<<dim3(CF::ROW_BLOCKS, sl.max_k, sl.count), CF::ROW_THREADS, 0, st>>>(sl.descs, sl.tw);
+  switch (sl.col_tc ? sl.col_tc : CF::TC) {
+    case 2: e = launch_col_adj<N, 2>(sl, st); break;
+    case 4: e = launch_col_adj<N, 4>(sl, st); break;
+    case 8: e = launch_col_adj<N, 8>(sl, st); break;
+    case 16: e = launch_col_adj<N, 16>(sl, st); break;
+    default: e = hipErrorInvalidValue;
+  }
+  if (e != hipSuccess) return e;
+  k_row_inv_add<N><<<dim3(CF::ROW_BLOCKS, sl.max_c, sl.count), CF::ROW_THREADS, 0, st>>>(sl.descs, gout, sl.tw);
+  constexpr int shift = ilog2c(CF::ROWS);
+  k_param_reduce<<<dim3(1, 1, sl.count), 256, 0, st>>>(sl.descs, shift);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint(int N, const StepLaunch& sl, int gin, int gout, hipStream_t st) {
+  switch (N) {
+    case 16: return launch_adjoint_n<16>(sl, gin, gout, st);
+    case 32: return launch_adjoint_n<32>(sl, gin, gout, st);
+    case 64: return launch_adjoint_n<64>(sl, gin, gout, st);
+    case 128: return launch_adjoint_n<128>(sl, gin, gout, st);
+    case 256: return launch_adjoint_n<256>(sl, gin, gout, st);
+    case 512: return launch_adjoint_n<512>(sl, gin, gout, st);
+    case 1024: return launch_adjoint_n<1024>(sl, gin, gout, st);
+    case 2048: return launch_adjoint_n<2048>(sl, gin, gout, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int row_blocks(int N) {
+  switch (N) {
+    case 16: return Cfg<16>::ROW_BLOCKS;
+    case 32: return Cfg<32>::ROW_BLOCKS;
+    case 64: return Cfg<64>::ROW_BLOCKS;
+    case 128: return Cfg<128>::ROW_BLOCKS;
+    case 256: return Cfg<256>::ROW_BLOCKS;
+    case 512: return Cfg<512>::ROW_BLOCKS;
+    case 1024: return Cfg<1024>::ROW_BLOCKS;
+    case 2048: return Cfg<2048>::ROW_BLOCKS;
+    default: return 0;
+  }
+}
+
+hipError_t launch_mse_grad(const CreatureDev* descs, int count, int sfin, int gout, hipStream_t st) {
+  k_mse_grad<<<dim3(kLossBlocks, 1, count), kLossThreads, 0, st>>>(descs, sfin, gout);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_gparam(const CreatureDev* descs, int count, hipStream_t st) {
+  k_zero_gparam<<<dim3(1, 1, count), 256, 0, st>>>(descs);
+  return hipGetLastError();
+}
+
+}  // namespace cn

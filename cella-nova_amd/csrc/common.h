@@ -12,6 +12,7 @@ namespace cn {
 
 cn_status set_error(cn_status st, const char* fmt, ...);
 void clear_error();
+const std::string& last_error_text();
 
 // Returns true (and records CN_ERR_HIP) if the call failed or a failure was injected.
 bool hip_failed(hipError_t err, const char* what, cn_status* st);

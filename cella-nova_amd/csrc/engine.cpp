@@ -1,7 +1,11 @@
-// Engine: the MI355X counterpart of the reference's GpuMaceLenia (ml-rs/orchestrator/mod.rs).
-// It owns the device buffers of one creature, builds the device descriptor, runs the step
-// (four kernels, captured once into two HIP graphs for the two ping-pong parities) and
-// implements the C ABI entry points of include/cellanova.h that act on an engine.
+// Engine: the MI355X counterpart of the reference's GpuMaceLenia (ml-rs/orchestrator/mod.rs),
+// generalised to a batch of creatures.  One engine owns the device buffers of one or more
+// creatures (each with its own N, C, K, maps and parameters), grouped into size classes (one
+// descriptor array and one launch per phase per class), one HIP stream, and HIP graphs:
+//   - the step graph (all classes, one per ping-pong parity) replayed by run_steps;
+//   - the rollout graph (forward T steps with checkpoints, MSE loss, checkpointed adjoint)
+//     captured once per (T, checkpoint stride) and replayed per optimisation iteration.
+// The single-creature C ABI of include/cellanova.h is the batch API with creature 0.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstdio>
@@ -30,6 +34,7 @@ cn_status set_error(cn_status st, const char* fmt, ...) {
   return st;
 }
 void clear_error() { g_err.clear(); }
+const std::string& last_error_text() { return g_err; }
 
 bool inject_failure() {
   if (g_fail_next > 0) {
@@ -47,7 +52,7 @@ bool hip_failed(hipError_t err, const char* what, cn_status* st) {
 }
 
 // ---------------------------------------------------------------------------------------
-// twiddles: tw[i] = exp(-2*pi*i*i/kTwN), FP64 then rounded once
+// host helpers
 // ---------------------------------------------------------------------------------------
 std::vector<float> make_twiddles() {
   std::vector<float> t(2 * kTwN);
@@ -59,6 +64,17 @@ std::vector<float> make_twiddles() {
   return t;
 }
 
+cn_status validate_growth(uint32_t K, const float* mu, const float* sigma, const float* w) {
+  if (!mu || !sigma || !w) return set_error(CN_ERR_INVALID_ARG, "null growth parameter array");
+  for (uint32_t k = 0; k < K; ++k) {
+    if (!std::isfinite(sigma[k]) || !(sigma[k] > 0.f))
+      return set_error(CN_ERR_INVALID_ARG, "sigma[%u] = %g must be finite and > 0", k, (double)sigma[k]);
+    if (!std::isfinite(mu[k])) return set_error(CN_ERR_INVALID_ARG, "mu[%u] is not finite", k);
+    if (!std::isfinite(w[k])) return set_error(CN_ERR_INVALID_ARG, "weights[%u] is not finite", k);
+  }
+  return CN_OK;
+}
+
 cn_status validate_creature(uint32_t n, uint32_t C, uint32_t K, const uint32_t* src, const uint32_t* dst,
                             const float* mu, const float* sigma, const float* w, float temp) {
   if (!is_pow2(n) || n < 16 || n > 2048)
@@ -67,17 +83,24 @@ cn_status validate_creature(uint32_t n, uint32_t C, uint32_t K, const uint32_t* 
     return set_error(CN_ERR_UNSUPPORTED, "num_channels %u must be in [1, %d]", C, kMaxChannels);
   if (K < 1 || K > (uint32_t)kMaxKernels)
     return set_error(CN_ERR_UNSUPPORTED, "num_kernels %u must be in [1, %d]", K, kMaxKernels);
-  if (!src || !dst || !mu || !sigma || !w) return set_error(CN_ERR_INVALID_ARG, "null parameter array");
+  if (!src || !dst) return set_error(CN_ERR_INVALID_ARG, "null src/dst array");
   for (uint32_t k = 0; k < K; ++k) {
     if (src[k] >= C) return set_error(CN_ERR_INVALID_ARG, "src[%u] = %u out of range (C = %u)", k, src[k], C);
     if (dst[k] >= C) return set_error(CN_ERR_INVALID_ARG, "dst[%u] = %u out of range (C = %u)", k, dst[k], C);
-    if (!std::isfinite(sigma[k]) || !(sigma[k] > 0.f))
-      return set_error(CN_ERR_INVALID_ARG, "sigma[%u] = %g must be finite and > 0", k, (double)sigma[k]);
-    if (!std::isfinite(mu[k])) return set_error(CN_ERR_INVALID_ARG, "mu[%u] is not finite", k);
-    if (!std::isfinite(w[k])) return set_error(CN_ERR_INVALID_ARG, "weights[%u] is not finite", k);
   }
+  const cn_status st = validate_growth(K, mu, sigma, w);
+  if (st != CN_OK) return st;
   if (!std::isfinite(temp)) return set_error(CN_ERR_INVALID_ARG, "temp is not finite");
   return CN_OK;
+}
+
+void fill_growth(CreatureDev& d, uint32_t K, const float* mu, const float* sigma, const float* w) {
+  for (uint32_t k = 0; k < K; ++k) {
+    d.mu[k] = mu[k];
+    d.sigma[k] = sigma[k];
+    d.inv_s2[k] = 1.0f / (sigma[k] * sigma[k]);  // example.rs:247
+    d.w[k] = w[k];
+  }
 }
 
 void fill_desc_params(CreatureDev& d, uint32_t C, uint32_t K, const uint32_t* src, const uint32_t* dst,
@@ -100,10 +123,10 @@ void fill_desc_params(CreatureDev& d, uint32_t C, uint32_t K, const uint32_t* sr
   }
   for (uint32_t c = C; c <= (uint32_t)kMaxChannels; ++c) d.dst_off[c] = pos;
   for (uint32_t k = 0; k < K; ++k) {
-    d.mu[k] = mu[k];
-    d.inv_s2[k] = 1.0f / (sigma[k] * sigma[k]);  // example.rs:247
-    d.w[k] = w[k];
+    d.src_of[k] = (int)src[k];
+    d.dst_of[k] = (int)dst[k];
   }
+  fill_growth(d, K, mu, sigma, w);
 }
 
 // Hermitian part of a full N*N spectrum, scaled by 1/N^2, split into the half layout:
@@ -128,80 +151,424 @@ void project_kernel(const float* full, uint32_t N, float* half, float* nyq) {
 
 using namespace cn;
 
+namespace {
+
+struct Creature {
+  uint32_t N = 0, C = 0, K = 0;
+  CreatureDev dev{};       // host copy of the device descriptor
+  int cls = 0, idx = 0;    // size class and position in its descriptor array
+  std::vector<void*> allocs;
+  bool adj = false;        // adjoint buffers allocated
+  bool has_target = false;
+  size_t plane() const { return (size_t)N * N; }
+  size_t slot_elems() const { return (size_t)C * N * N; }
+};
+
+struct SizeClass {
+  int N = 0;
+  std::vector<int> members;
+  CreatureDev* d_descs = nullptr;
+  int max_c = 0, max_k = 0;
+};
+
+struct Rollout {
+  uint32_t T = 0, ckpt = 0;
+  hipGraphExec_t graph = nullptr;   // forward + loss + backward (targets)
+  hipGraphExec_t fwd = nullptr;     // forward only
+  int final_slot = 0;
+  int grad_slot = 0;                // gradient slot holding dL/dstate0 after the backward
+};
+
+}  // namespace
+
 struct cn_engine {
   int device = 0;
   hipStream_t stream = nullptr;
-  uint32_t N = 0, C = 0, K = 0;
-  int cur = 0;  // state[cur] holds the current world
+  std::vector<Creature> cr;
+  std::vector<SizeClass> cls;
+  int nslots = 2;
+  int cur = 0;              // current-state slot (0 or 1)
   uint64_t steps = 0;
-  bool have_prev = false;  // state[cur ^ 1] holds the input of the most recent step
+  bool have_prev = false;   // slot cur ^ 1 holds the input of the most recent step
   uint64_t bytes = 0;
-  CreatureDev host_desc{};
-  CreatureDev* d_desc = nullptr;
   float2* tw = nullptr;
   double2* mass_partial = nullptr;
-  int mass_nblk = 0;
+  int mass_nblk = 64;
   std::vector<void*> allocs;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t step_graph[2] = {nullptr, nullptr};
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  int col_tc = 0;  // fused column pass tile width (0: built-in default for N)
+  int col_tc = 0;           // fused column pass tile width (0: built-in default for N)
+  Rollout ro;               // last rollout plan
+  bool ro_fwd_done = false; // a rollout forward has run since the last state change
 };
 
 namespace {
 
-cn_status dalloc(cn_engine* e, void** p, size_t bytes) {
+cn_status dalloc(cn_engine* e, std::vector<void*>& owner, void** p, size_t bytes) {
+  if (inject_failure()) return set_error(CN_ERR_HIP, "hipMalloc: injected failure");
   CN_HIP(hipMalloc(p, bytes));
-  e->allocs.push_back(*p);
+  owner.push_back(*p);
   e->bytes += bytes;
   CN_HIP(hipMemsetAsync(*p, 0, bytes, e->stream));
   return CN_OK;
 }
 
+void drop_graphs(cn_engine* e) {
+  for (auto& g : e->step_graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  if (e->ro.graph) (void)hipGraphExecDestroy(e->ro.graph);
+  if (e->ro.fwd) (void)hipGraphExecDestroy(e->ro.fwd);
+  e->ro = Rollout{};
+}
+
 void release(cn_engine* e) {
   if (!e) return;
   (void)hipSetDevice(e->device);
-  for (auto& g : e->graph)
-    if (g) (void)hipGraphExecDestroy(g);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  drop_graphs(e);
   for (auto& v : e->ev)
     if (v) (void)hipEventDestroy(v);
-  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  for (auto& c : e->cr)
+    for (void* p : c.allocs) (void)hipFree(p);
   for (void* p : e->allocs) (void)hipFree(p);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }
 
-StepLaunch base_launch(cn_engine* e, int parity) {
+cn_status upload_desc(cn_engine* e, int i) {
+  Creature& c = e->cr[i];
+  CN_HIP(hipMemcpyAsync(e->cls[c.cls].d_descs + c.idx, &c.dev, sizeof(CreatureDev), hipMemcpyHostToDevice,
+                        e->stream));
+  CN_HIP(hipStreamSynchronize(e->stream));
+  return CN_OK;
+}
+
+StepLaunch class_launch(cn_engine* e, const SizeClass& k, int sin, int sout) {
   StepLaunch sl;
-  sl.descs = e->d_desc;
-  sl.count = 1;
-  sl.parity = parity;
-  sl.max_c = (int)e->C;
+  sl.descs = k.d_descs;
+  sl.count = (int)k.members.size();
+  sl.sin = sin;
+  sl.sout = sout;
+  sl.max_c = k.max_c;
+  sl.max_k = k.max_k;
   sl.tw = e->tw;
-  sl.col_tc = e->col_tc;
+  sl.col_tc = (k.N == (int)e->cr[0].N) ? e->col_tc : 0;
   return sl;
 }
 
-cn_status enqueue_step(cn_engine* e) {
-  if (!e->graph[e->cur]) {
-    hipGraph_t g = nullptr;
-    CN_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    const hipError_t le = launch_step((int)e->N, base_launch(e, e->cur), e->stream);
-    const hipError_t ce = hipStreamEndCapture(e->stream, &g);
-    cn_status st;
-    if (hip_failed(le, "launch_step (capture)", &st)) {
-      if (g) (void)hipGraphDestroy(g);
-      return st;
-    }
-    if (hip_failed(ce, "hipStreamEndCapture", &st)) return st;
-    const hipError_t ie = hipGraphInstantiate(&e->graph[e->cur], g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (hip_failed(ie, "hipGraphInstantiate", &st)) return st;
+hipError_t launch_all_steps(cn_engine* e, int sin, int sout) {
+  for (const SizeClass& k : e->cls) {
+    const hipError_t err = launch_step(k.N, class_launch(e, k, sin, sout), e->stream);
+    if (err != hipSuccess) return err;
   }
-  CN_HIP(hipGraphLaunch(e->graph[e->cur], e->stream));
+  return hipSuccess;
+}
+
+hipError_t launch_all_adjoint(cn_engine* e, int sin, int gin, int gout) {
+  for (const SizeClass& k : e->cls) {
+    const hipError_t err = launch_adjoint(k.N, class_launch(e, k, sin, sin), gin, gout, e->stream);
+    if (err != hipSuccess) return err;
+  }
+  return hipSuccess;
+}
+
+// Captures `fn` (a sequence of launches on e->stream) into an executable graph.
+template <class F>
+cn_status capture(cn_engine* e, hipGraphExec_t* out, F&& fn) {
+  hipGraph_t g = nullptr;
+  CN_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  const hipError_t le = fn();
+  const hipError_t ce = hipStreamEndCapture(e->stream, &g);
+  cn_status st;
+  if (hip_failed(le, "kernel launch (capture)", &st)) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (hip_failed(ce, "hipStreamEndCapture", &st)) return st;
+  const hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (hip_failed(ie, "hipGraphInstantiate", &st)) return st;
+  return CN_OK;
+}
+
+cn_status enqueue_step(cn_engine* e) {
+  const int cur = e->cur;
+  if (!e->step_graph[cur]) {
+    const cn_status st = capture(e, &e->step_graph[cur], [&] { return launch_all_steps(e, cur, cur ^ 1); });
+    if (st != CN_OK) return st;
+  }
+  CN_HIP(hipGraphLaunch(e->step_graph[cur], e->stream));
   e->cur ^= 1;
   e->steps++;
   e->have_prev = true;
+  e->ro_fwd_done = false;
   return CN_OK;
+}
+
+cn_status check_creature(const cn_engine* e, uint32_t i) {
+  if (!e) return set_error(CN_ERR_INVALID_ARG, "null engine");
+  if (i >= e->cr.size()) return set_error(CN_ERR_INVALID_ARG, "creature %u out of range (%zu)", i, e->cr.size());
+  return CN_OK;
+}
+
+// Grows every creature's state-slot array to `want` slots (slots 0/1 are preserved).
+cn_status ensure_slots(cn_engine* e, int want) {
+  if (want <= e->nslots) return CN_OK;
+  CN_HIP(hipStreamSynchronize(e->stream));
+  for (size_t i = 0; i < e->cr.size(); ++i) {
+    Creature& c = e->cr[i];
+    void* p = nullptr;
+    const cn_status st = dalloc(e, c.allocs, &p, (size_t)want * c.slot_elems() * 4);
+    if (st != CN_OK) return st;
+    CN_HIP(hipMemcpyAsync(p, c.dev.state, 2 * c.slot_elems() * 4, hipMemcpyDeviceToDevice, e->stream));
+    CN_HIP(hipStreamSynchronize(e->stream));
+    auto it = std::find(c.allocs.begin(), c.allocs.end(), (void*)c.dev.state);
+    if (it != c.allocs.end()) {
+      CN_HIP(hipFree(*it));
+      e->bytes -= (size_t)e->nslots * c.slot_elems() * 4;
+      c.allocs.erase(it);
+    }
+    c.dev.state = (float*)p;
+    c.dev.nslots = want;
+    const cn_status s2 = upload_desc(e, (int)i);
+    if (s2 != CN_OK) return s2;
+  }
+  e->nslots = want;
+  return CN_OK;
+}
+
+cn_status ensure_adjoint(cn_engine* e) {
+  for (size_t i = 0; i < e->cr.size(); ++i) {
+    Creature& c = e->cr[i];
+    if (c.adj) continue;
+    const size_t P = c.plane(), C = c.C, K = c.K, H = c.N / 2;
+    CreatureDev& d = c.dev;
+    void* p = nullptr;
+    cn_status st;
+    if (!d.u) {
+      if ((st = dalloc(e, c.allocs, &p, K * P * 4)) != CN_OK) return st;
+      d.u = (float*)p;
+    }
+    if ((st = dalloc(e, c.allocs, &p, 2 * C * P * 4)) != CN_OK) return st;
+    d.grad = (float*)p;
+    if ((st = dalloc(e, c.allocs, &p, C * P * 4)) != CN_OK) return st;
+    d.gaff = (float*)p;
+    if ((st = dalloc(e, c.allocs, &p, C * P * 4)) != CN_OK) return st;
+    d.dsd = (float*)p;
+    if ((st = dalloc(e, c.allocs, &p, K * c.N * H * 8)) != CN_OK) return st;
+    d.specg = p;
+    if ((st = dalloc(e, c.allocs, &p, (size_t)row_blocks((int)c.N) * K * 3 * 4)) != CN_OK) return st;
+    d.partial = (float*)p;
+    if ((st = dalloc(e, c.allocs, &p, 6 * K * 4)) != CN_OK) return st;
+    d.gparam = (float*)p;
+    if ((st = dalloc(e, c.allocs, &p, C * P * 4)) != CN_OK) return st;
+    d.target = (float*)p;
+    if ((st = dalloc(e, c.allocs, &p, (size_t)kLossPartials * 4)) != CN_OK) return st;
+    d.lossp = (float*)p;
+    c.adj = true;
+    if ((st = upload_desc(e, (int)i)) != CN_OK) return st;
+  }
+  return CN_OK;
+}
+
+// Slot plan of a rollout of T steps with a checkpoint every `ck` steps, starting from slot
+// `start` (0 or 1, left unmodified):
+//   ck[0] = start; ck[j] (1 <= j < nck) = slot 2 + j - 1; forward scratch X0, X1;
+//   recompute buffers seg[i] (1 <= i < ck); final state.
+struct SlotPlan {
+  int nck, x0, seg0, fin, total;
+  int ckslot(int j, int start) const { return j == 0 ? start : 2 + j - 1; }
+  int segslot(int i) const { return seg0 + i - 1; }
+};
+SlotPlan plan_slots(uint32_t T, uint32_t ck) {
+  SlotPlan p;
+  p.nck = (int)((T + ck - 1) / ck);
+  p.x0 = 2 + p.nck - 1;
+  p.seg0 = p.x0 + 2;
+  p.fin = p.seg0 + (int)ck - 1;
+  p.total = p.fin + 1;
+  return p;
+}
+
+hipError_t record_forward(cn_engine* e, const SlotPlan& p, uint32_t T, uint32_t ck, int start) {
+  int in = start;
+  for (uint32_t t = 0; t < T; ++t) {
+    int out;
+    if (t + 1 == T) out = p.fin;
+    else if ((t + 1) % ck == 0) out = p.ckslot((int)((t + 1) / ck), start);
+    else out = p.x0 + (int)(t % 2);
+    const hipError_t err = launch_all_steps(e, in, out);
+    if (err != hipSuccess) return err;
+    in = out;
+  }
+  return hipSuccess;
+}
+
+// Backward from gradient slot 0 (dL/dfinal); returns the slot holding dL/dstate0.
+hipError_t record_backward(cn_engine* e, const SlotPlan& p, uint32_t T, uint32_t ck, int start, int* gslot) {
+  hipError_t err;
+  for (const SizeClass& k : e->cls)
+    if ((err = launch_zero_gparam(k.d_descs, (int)k.members.size(), e->stream)) != hipSuccess) return err;
+  int g = 0;
+  for (int j = p.nck - 1; j >= 0; --j) {
+    const uint32_t t0 = (uint32_t)j * ck, t1 = std::min(T, t0 + ck);
+    auto state_slot = [&](uint32_t t) { return t == t0 ? p.ckslot(j, start) : p.segslot((int)(t - t0)); };
+    for (uint32_t t = t0 + 1; t < t1; ++t)
+      if ((err = launch_all_steps(e, state_slot(t - 1), state_slot(t))) != hipSuccess) return err;
+    for (uint32_t t = t1; t-- > t0;) {
+      if ((err = launch_all_adjoint(e, state_slot(t), g, g ^ 1)) != hipSuccess) return err;
+      g ^= 1;
+    }
+  }
+  *gslot = g;
+  return hipSuccess;
+}
+
+cn_status prepare_rollout(cn_engine* e, uint32_t T, uint32_t ck) {
+  if (T == 0 || ck == 0) return set_error(CN_ERR_INVALID_ARG, "steps and checkpoint stride must be > 0");
+  if (ck > T) ck = T;
+  cn_status st;
+  if ((st = ensure_adjoint(e)) != CN_OK) return st;
+  const SlotPlan p = plan_slots(T, ck);
+  if ((st = ensure_slots(e, p.total)) != CN_OK) return st;
+  if (e->ro.T == T && e->ro.ckpt == ck && e->ro.graph && e->ro.fwd) return CN_OK;
+  if (e->ro.graph) (void)hipGraphExecDestroy(e->ro.graph);
+  if (e->ro.fwd) (void)hipGraphExecDestroy(e->ro.fwd);
+  e->ro = Rollout{};
+  // Graphs always start from slot 0: the engine moves its current state there first.
+  if ((st = capture(e, &e->ro.fwd, [&] { return record_forward(e, p, T, ck, 0); })) != CN_OK) return st;
+  int gslot = 0;
+  st = capture(e, &e->ro.graph, [&] {
+    hipError_t err = record_forward(e, p, T, ck, 0);
+    if (err != hipSuccess) return err;
+    for (const SizeClass& k : e->cls)
+      if ((err = launch_mse_grad(k.d_descs, (int)k.members.size(), p.fin, 0, e->stream)) != hipSuccess) return err;
+    return record_backward(e, p, T, ck, 0, &gslot);
+  });
+  if (st != CN_OK) return st;
+  e->ro.T = T;
+  e->ro.ckpt = ck;
+  e->ro.final_slot = p.fin;
+  e->ro.grad_slot = gslot;
+  return CN_OK;
+}
+
+// Rollouts start from slot 0; move the current state there if it is in slot 1.
+cn_status normalise_current(cn_engine* e) {
+  if (e->cur == 0) return CN_OK;
+  for (auto& c : e->cr)
+    CN_HIP(hipMemcpyAsync(c.dev.state, c.dev.state + c.slot_elems(), c.slot_elems() * 4, hipMemcpyDeviceToDevice,
+                          e->stream));
+  e->cur = 0;
+  e->have_prev = false;
+  return CN_OK;
+}
+
+cn_status create_engine(const cn_engine_desc* descs, uint32_t count, int device, cn_engine** out) {
+  if (!descs || !out) return set_error(CN_ERR_INVALID_ARG, "null descs or out");
+  *out = nullptr;
+  if (count == 0) return set_error(CN_ERR_INVALID_ARG, "creature count must be > 0");
+  for (uint32_t i = 0; i < count; ++i) {
+    const cn_engine_desc& d = descs[i];
+    const cn_status st =
+        validate_creature(d.n, d.num_channels, d.num_kernels, d.src, d.dst, d.mu, d.sigma, d.weights, d.temp);
+    if (st != CN_OK) {
+      if (count > 1) {
+        const std::string msg = last_error_text();
+        set_error(st, "creature %u: %s", i, msg.c_str());
+      }
+      return st;
+    }
+  }
+  if (device < 0) return set_error(CN_ERR_INVALID_ARG, "device %d", device);
+
+  cn_engine* e = new cn_engine();
+  e->device = device;
+  if (const char* tc = std::getenv("CN_COL_TC")) e->col_tc = std::atoi(tc);
+  auto fail = [&](cn_status s) {
+    release(e);
+    return s;
+  };
+  cn_status st;
+  if (inject_failure()) return fail(set_error(CN_ERR_HIP, "hipSetDevice: injected failure"));
+  if (hip_failed(hipSetDevice(device), "hipSetDevice", &st)) return fail(st);
+  if (hip_failed(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate", &st)) return fail(st);
+  for (auto& v : e->ev)
+    if (hip_failed(hipEventCreate(&v), "hipEventCreate", &st)) return fail(st);
+
+  // size classes in order of first appearance
+  e->cr.resize(count);
+  for (uint32_t i = 0; i < count; ++i) {
+    const cn_engine_desc& d = descs[i];
+    Creature& c = e->cr[i];
+    c.N = d.n;
+    c.C = d.num_channels;
+    c.K = d.num_kernels;
+    int ci = -1;
+    for (size_t j = 0; j < e->cls.size(); ++j)
+      if (e->cls[j].N == (int)d.n) ci = (int)j;
+    if (ci < 0) {
+      e->cls.push_back(SizeClass{});
+      ci = (int)e->cls.size() - 1;
+      e->cls[ci].N = (int)d.n;
+    }
+    SizeClass& k = e->cls[ci];
+    c.cls = ci;
+    c.idx = (int)k.members.size();
+    k.members.push_back((int)i);
+    k.max_c = std::max(k.max_c, (int)c.C);
+    k.max_k = std::max(k.max_k, (int)c.K);
+  }
+  for (uint32_t i = 0; i < count; ++i) {
+    const cn_engine_desc& dd = descs[i];
+    Creature& c = e->cr[i];
+    const size_t N = c.N, C = c.C, K = c.K, P = N * N, H = N / 2;
+    CreatureDev& d = c.dev;
+    d.N = (int)N;
+    void* p = nullptr;
+    if ((st = dalloc(e, c.allocs, &p, 2 * C * P * 4)) != CN_OK) return fail(st);
+    d.state = (float*)p;
+    d.nslots = 2;
+    if ((st = dalloc(e, c.allocs, &p, C * N * H * 8)) != CN_OK) return fail(st);
+    d.spec = p;
+    if ((st = dalloc(e, c.allocs, &p, K * N * H * 8)) != CN_OK) return fail(st);
+    d.khalf = p;
+    if ((st = dalloc(e, c.allocs, &p, K * N * 8)) != CN_OK) return fail(st);
+    d.knyq = p;
+    if ((st = dalloc(e, c.allocs, &p, K * N * H * 8)) != CN_OK) return fail(st);
+    d.uhat = p;
+    if ((st = dalloc(e, c.allocs, &p, C * P * 4)) != CN_OK) return fail(st);
+    d.A = (float*)p;
+    fill_desc_params(d, c.C, c.K, dd.src, dd.dst, dd.mu, dd.sigma, dd.weights, dd.temp);
+  }
+  for (SizeClass& k : e->cls) {
+    void* p = nullptr;
+    if ((st = dalloc(e, e->allocs, &p, k.members.size() * sizeof(CreatureDev))) != CN_OK) return fail(st);
+    k.d_descs = (CreatureDev*)p;
+  }
+  for (uint32_t i = 0; i < count; ++i)
+    if ((st = upload_desc(e, (int)i)) != CN_OK) return fail(st);
+  void* p = nullptr;
+  if ((st = dalloc(e, e->allocs, &p, (size_t)kTwN * 8)) != CN_OK) return fail(st);
+  e->tw = (float2*)p;
+  const std::vector<float> tw = make_twiddles();
+  if ((st = dalloc(e, e->allocs, &p, (size_t)count * kMaxChannels * e->mass_nblk * sizeof(double2))) != CN_OK)
+    return fail(st);
+  e->mass_partial = (double2*)p;
+  if (hip_failed(hipMemcpyAsync(e->tw, tw.data(), tw.size() * 4, hipMemcpyHostToDevice, e->stream),
+                 "hipMemcpy twiddles", &st))
+    return fail(st);
+  if (hip_failed(hipStreamSynchronize(e->stream), "hipStreamSynchronize", &st)) return fail(st);
+  *out = e;
+  return CN_OK;
+}
+
+float* cur_state(cn_engine* e, uint32_t i) {
+  Creature& c = e->cr[i];
+  return c.dev.state + (size_t)e->cur * c.slot_elems();
 }
 
 }  // namespace
@@ -214,76 +581,13 @@ void cn_debug_fail_next_hip(int count) { g_fail_next = count > 0 ? count : 0; }
 
 cn_status cn_engine_create(const cn_engine_desc* desc, cn_engine** out) {
   clear_error();
-  if (!desc || !out) return set_error(CN_ERR_INVALID_ARG, "null desc or out");
-  *out = nullptr;
-  cn_status st = validate_creature(desc->n, desc->num_channels, desc->num_kernels, desc->src, desc->dst,
-                                   desc->mu, desc->sigma, desc->weights, desc->temp);
-  if (st != CN_OK) return st;
-  if (desc->device < 0) return set_error(CN_ERR_INVALID_ARG, "device %d", desc->device);
+  if (!desc) return set_error(CN_ERR_INVALID_ARG, "null desc or out");
+  return create_engine(desc, 1, desc->device, out);
+}
 
-  cn_engine* e = new cn_engine();
-  e->device = desc->device;
-  e->N = desc->n;
-  if (const char* tc = std::getenv("CN_COL_TC")) e->col_tc = std::atoi(tc);
-  e->C = desc->num_channels;
-  e->K = desc->num_kernels;
-  auto fail = [&](cn_status s) {
-    release(e);
-    return s;
-  };
-  {
-    cn_status s2;
-    if (inject_failure()) return fail(set_error(CN_ERR_HIP, "hipSetDevice: injected failure"));
-    if (hip_failed(hipSetDevice(e->device), "hipSetDevice", &s2)) return fail(s2);
-    if (hip_failed(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate", &s2))
-      return fail(s2);
-    for (auto& v : e->ev)
-      if (hip_failed(hipEventCreate(&v), "hipEventCreate", &s2)) return fail(s2);
-  }
-  const size_t N = e->N, C = e->C, K = e->K, P = N * N, H = N / 2;
-  CreatureDev& d = e->host_desc;
-  d.N = (int)N;
-  void* p = nullptr;
-  for (int s = 0; s < 2; ++s) {
-    if ((st = dalloc(e, &p, C * P * 4)) != CN_OK) return fail(st);
-    d.state[s] = (float*)p;
-  }
-  if ((st = dalloc(e, &p, C * N * H * 8)) != CN_OK) return fail(st);
-  d.spec = p;
-  if ((st = dalloc(e, &p, K * N * H * 8)) != CN_OK) return fail(st);
-  d.khalf = p;
-  if ((st = dalloc(e, &p, K * N * 8)) != CN_OK) return fail(st);
-  d.knyq = p;
-  if ((st = dalloc(e, &p, K * N * H * 8)) != CN_OK) return fail(st);
-  d.uhat = p;
-  if ((st = dalloc(e, &p, C * P * 4)) != CN_OK) return fail(st);
-  d.A = (float*)p;
-  d.u = nullptr;  // allocated on first conv readback
-  fill_desc_params(d, e->C, e->K, desc->src, desc->dst, desc->mu, desc->sigma, desc->weights, desc->temp);
-  if ((st = dalloc(e, &p, sizeof(CreatureDev))) != CN_OK) return fail(st);
-  e->d_desc = (CreatureDev*)p;
-  {
-    cn_status s2;
-    if (inject_failure()) return fail(set_error(CN_ERR_HIP, "hipMemcpy: injected failure"));
-    if (hip_failed(hipMemcpyAsync(e->d_desc, &d, sizeof d, hipMemcpyHostToDevice, e->stream), "hipMemcpy desc",
-                   &s2))
-      return fail(s2);
-  }
-  if ((st = dalloc(e, &p, (size_t)kTwN * 8)) != CN_OK) return fail(st);
-  e->tw = (float2*)p;
-  const std::vector<float> tw = make_twiddles();
-  e->mass_nblk = 64;
-  if ((st = dalloc(e, &p, (size_t)kMaxChannels * e->mass_nblk * sizeof(double2))) != CN_OK) return fail(st);
-  e->mass_partial = (double2*)p;
-  {
-    cn_status s2;
-    if (hip_failed(hipMemcpyAsync(e->tw, tw.data(), tw.size() * 4, hipMemcpyHostToDevice, e->stream),
-                   "hipMemcpy twiddles", &s2))
-      return fail(s2);
-    if (hip_failed(hipStreamSynchronize(e->stream), "hipStreamSynchronize", &s2)) return fail(s2);
-  }
-  *out = e;
-  return CN_OK;
+cn_status cn_batch_create(const cn_engine_desc* descs, uint32_t count, int device, cn_engine** out) {
+  clear_error();
+  return create_engine(descs, count, device, out);
 }
 
 cn_status cn_engine_destroy(cn_engine* e) {
@@ -293,57 +597,78 @@ cn_status cn_engine_destroy(cn_engine* e) {
   return CN_OK;
 }
 
-cn_status cn_set_kernel(cn_engine* e, uint32_t k, const float* interleaved, size_t n_complex) {
+cn_status cn_creature_set_kernel(cn_engine* e, uint32_t i, uint32_t k, const float* interleaved, size_t n_complex) {
   clear_error();
-  if (!e || !interleaved) return set_error(CN_ERR_INVALID_ARG, "null argument");
-  if (k >= e->K) return set_error(CN_ERR_INVALID_ARG, "kernel index %u out of range (K = %u)", k, e->K);
-  const size_t N = e->N, H = N / 2;
+  cn_status st = check_creature(e, i);
+  if (st != CN_OK) return st;
+  if (!interleaved) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  Creature& c = e->cr[i];
+  if (k >= c.K) return set_error(CN_ERR_INVALID_ARG, "kernel index %u out of range (K = %u)", k, c.K);
+  const size_t N = c.N, H = N / 2;
   if (n_complex != N * N)
     return set_error(CN_ERR_INVALID_ARG, "kernel spectrum has %zu values, expected %zu", n_complex, N * N);
   if (!finite_all(interleaved, 2 * n_complex)) return set_error(CN_ERR_INVALID_ARG, "kernel spectrum not finite");
   std::vector<float> half(N * H * 2), nyq(N * 2);
-  project_kernel(interleaved, e->N, half.data(), nyq.data());
+  project_kernel(interleaved, c.N, half.data(), nyq.data());
   CN_HIP(hipSetDevice(e->device));
-  const CreatureDev& d = e->host_desc;
-  CN_HIP(hipMemcpyAsync((char*)d.khalf + (size_t)k * N * H * 8, half.data(), half.size() * 4,
+  CN_HIP(hipMemcpyAsync((char*)c.dev.khalf + (size_t)k * N * H * 8, half.data(), half.size() * 4,
                         hipMemcpyHostToDevice, e->stream));
-  CN_HIP(hipMemcpyAsync((char*)d.knyq + (size_t)k * N * 8, nyq.data(), nyq.size() * 4, hipMemcpyHostToDevice,
+  CN_HIP(hipMemcpyAsync((char*)c.dev.knyq + (size_t)k * N * 8, nyq.data(), nyq.size() * 4, hipMemcpyHostToDevice,
                         e->stream));
   CN_HIP(hipStreamSynchronize(e->stream));
   return CN_OK;
 }
 
-static cn_status upload_f32(cn_engine* e, uint32_t c, const float* data) {
+cn_status cn_set_kernel(cn_engine* e, uint32_t k, const float* interleaved, size_t n_complex) {
+  return cn_creature_set_kernel(e, 0, k, interleaved, n_complex);
+}
+
+cn_status cn_creature_upload_channel_f32(cn_engine* e, uint32_t i, uint32_t channel, const float* data, size_t n) {
+  clear_error();
+  cn_status st = check_creature(e, i);
+  if (st != CN_OK) return st;
+  if (!data) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  Creature& c = e->cr[i];
+  if (channel >= c.C) return set_error(CN_ERR_INVALID_ARG, "channel %u out of range (C = %u)", channel, c.C);
+  const size_t P = c.plane();
+  if (n != P) return set_error(CN_ERR_INVALID_ARG, "channel has %zu values, expected %zu", n, P);
+  if (!finite_all(data, P)) return set_error(CN_ERR_INVALID_ARG, "channel values not finite");
   CN_HIP(hipSetDevice(e->device));
-  const size_t P = (size_t)e->N * e->N;
-  CN_HIP(hipMemcpyAsync(e->host_desc.state[e->cur] + c * P, data, P * 4, hipMemcpyHostToDevice, e->stream));
+  CN_HIP(hipMemcpyAsync(cur_state(e, i) + channel * P, data, P * 4, hipMemcpyHostToDevice, e->stream));
   CN_HIP(hipStreamSynchronize(e->stream));
   e->have_prev = false;
+  e->ro_fwd_done = false;
   return CN_OK;
+}
+
+cn_status cn_upload_channel_f32(cn_engine* e, uint32_t channel, const float* data, size_t n) {
+  return cn_creature_upload_channel_f32(e, 0, channel, data, n);
 }
 
 cn_status cn_upload_channel_f64(cn_engine* e, uint32_t channel, const double* data, size_t n) {
   clear_error();
-  if (!e || !data) return set_error(CN_ERR_INVALID_ARG, "null argument");
-  if (channel >= e->C) return set_error(CN_ERR_INVALID_ARG, "channel %u out of range (C = %u)", channel, e->C);
-  const size_t P = (size_t)e->N * e->N;
-  if (n != P) return set_error(CN_ERR_INVALID_ARG, "channel has %zu values, expected %zu", n, P);
-  std::vector<float> f(P);
-  for (size_t i = 0; i < P; ++i) {
-    if (!std::isfinite(data[i])) return set_error(CN_ERR_INVALID_ARG, "channel value %zu not finite", i);
-    f[i] = (float)data[i];  // mod.rs:220
+  cn_status st = check_creature(e, 0);
+  if (st != CN_OK) return st;
+  if (!data) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  std::vector<float> f(n);
+  for (size_t j = 0; j < n; ++j) {
+    if (!std::isfinite(data[j])) return set_error(CN_ERR_INVALID_ARG, "channel value %zu not finite", j);
+    f[j] = (float)data[j];  // mod.rs:220
   }
-  return upload_f32(e, channel, f.data());
+  return cn_creature_upload_channel_f32(e, 0, channel, f.data(), n);
 }
 
-cn_status cn_upload_channel_f32(cn_engine* e, uint32_t channel, const float* data, size_t n) {
+cn_status cn_creature_set_growth(cn_engine* e, uint32_t i, const float* mu, const float* sigma, const float* w,
+                                 size_t K) {
   clear_error();
-  if (!e || !data) return set_error(CN_ERR_INVALID_ARG, "null argument");
-  if (channel >= e->C) return set_error(CN_ERR_INVALID_ARG, "channel %u out of range (C = %u)", channel, e->C);
-  const size_t P = (size_t)e->N * e->N;
-  if (n != P) return set_error(CN_ERR_INVALID_ARG, "channel has %zu values, expected %zu", n, P);
-  if (!finite_all(data, P)) return set_error(CN_ERR_INVALID_ARG, "channel values not finite");
-  return upload_f32(e, channel, data);
+  cn_status st = check_creature(e, i);
+  if (st != CN_OK) return st;
+  Creature& c = e->cr[i];
+  if (K != c.K) return set_error(CN_ERR_INVALID_ARG, "expected %u growth parameters, got %zu", c.K, K);
+  if ((st = validate_growth(c.K, mu, sigma, w)) != CN_OK) return st;
+  fill_growth(c.dev, c.K, mu, sigma, w);
+  CN_HIP(hipSetDevice(e->device));
+  return upload_desc(e, (int)i);
 }
 
 cn_status cn_iterate(cn_engine* e) {
@@ -357,7 +682,7 @@ cn_status cn_run_steps(cn_engine* e, uint64_t n) {
   clear_error();
   if (!e) return set_error(CN_ERR_INVALID_ARG, "null engine");
   CN_HIP(hipSetDevice(e->device));
-  for (uint64_t i = 0; i < n; ++i) {
+  for (uint64_t j = 0; j < n; ++j) {
     const cn_status st = enqueue_step(e);
     if (st != CN_OK) return st;
   }
@@ -372,17 +697,23 @@ cn_status cn_sync(cn_engine* e) {
   return CN_OK;
 }
 
-// One directly launched step with events around every kernel; kernel_ms receives
-// [row forward, fused column, inverse row + growth, diffusion] in milliseconds.
+// One directly launched step of every creature with events around every kernel of the first
+// size class; kernel_ms receives [row forward, fused column, inverse row + growth, diffusion].
 static cn_status timed_step(cn_engine* e, float kernel_ms[4]) {
-  StepLaunch sl = base_launch(e, e->cur);
-  for (int i = 0; i < 5; ++i) sl.ev[i] = e->ev[i];
-  CN_HIP(launch_step((int)e->N, sl, e->stream));
+  const int cur = e->cur;
+  for (size_t j = 0; j < e->cls.size(); ++j) {
+    StepLaunch sl = class_launch(e, e->cls[j], cur, cur ^ 1);
+    if (j == 0)
+      for (int q = 0; q < 5; ++q) sl.ev[q] = e->ev[q];
+    CN_HIP(launch_step(e->cls[j].N, sl, e->stream));
+  }
   CN_HIP(hipEventSynchronize(e->ev[4]));
-  for (int i = 0; i < 4; ++i) CN_HIP(hipEventElapsedTime(&kernel_ms[i], e->ev[i], e->ev[i + 1]));
+  for (int q = 0; q < 4; ++q) CN_HIP(hipEventElapsedTime(&kernel_ms[q], e->ev[q], e->ev[q + 1]));
+  CN_HIP(hipStreamSynchronize(e->stream));
   e->cur ^= 1;
   e->steps++;
   e->have_prev = true;
+  e->ro_fwd_done = false;
   return CN_OK;
 }
 
@@ -411,88 +742,255 @@ cn_status cn_profile_kernels(cn_engine* e, uint64_t steps, double* kernel_ms, si
     float k[4];
     const cn_status st = timed_step(e, k);
     if (st != CN_OK) return st;
-    for (int i = 0; i < 4; ++i) acc[i] += k[i];
+    for (int q = 0; q < 4; ++q) acc[q] += k[q];
   }
-  for (int i = 0; i < 4; ++i) kernel_ms[i] = acc[i] / (double)steps;
+  for (int q = 0; q < 4; ++q) kernel_ms[q] = acc[q] / (double)steps;
   return CN_OK;
 }
 
-cn_status cn_download_channels(cn_engine* e, float* out, size_t n) {
+cn_status cn_creature_download_channels(cn_engine* e, uint32_t i, float* out, size_t n) {
   clear_error();
-  if (!e || !out) return set_error(CN_ERR_INVALID_ARG, "null argument");
-  const size_t tot = (size_t)e->C * e->N * e->N;
+  cn_status st = check_creature(e, i);
+  if (st != CN_OK) return st;
+  if (!out) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  const size_t tot = e->cr[i].slot_elems();
   if (n != tot) return set_error(CN_ERR_INVALID_ARG, "output has %zu values, expected %zu", n, tot);
   CN_HIP(hipSetDevice(e->device));
-  CN_HIP(hipMemcpyAsync(out, e->host_desc.state[e->cur], tot * 4, hipMemcpyDeviceToHost, e->stream));
+  CN_HIP(hipMemcpyAsync(out, cur_state(e, i), tot * 4, hipMemcpyDeviceToHost, e->stream));
   CN_HIP(hipStreamSynchronize(e->stream));
   return CN_OK;
 }
 
+cn_status cn_download_channels(cn_engine* e, float* out, size_t n) {
+  return cn_creature_download_channels(e, 0, out, n);
+}
+
 cn_status cn_download_conv_real(cn_engine* e, float* out, size_t n) {
   clear_error();
-  if (!e || !out) return set_error(CN_ERR_INVALID_ARG, "null argument");
-  const size_t P = (size_t)e->N * e->N, tot = (size_t)e->K * P;
+  cn_status st = check_creature(e, 0);
+  if (st != CN_OK) return st;
+  if (!out) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  Creature& c = e->cr[0];
+  const size_t P = c.plane(), tot = (size_t)c.K * P;
   if (n != tot) return set_error(CN_ERR_INVALID_ARG, "output has %zu values, expected %zu", n, tot);
   if (!e->have_prev) {
     if (e->steps == 0) {
       std::memset(out, 0, tot * 4);
       return CN_OK;
     }
-    return set_error(CN_ERR_STATE, "the input of the last step was overwritten by an upload");
+    return set_error(CN_ERR_STATE, "the input of the last step was overwritten since it ran");
   }
   CN_HIP(hipSetDevice(e->device));
-  CreatureDev& d = e->host_desc;
-  if (!d.u) {
+  if (!c.dev.u) {
     void* p = nullptr;
-    const cn_status st = dalloc(e, &p, tot * 4);
-    if (st != CN_OK) return st;
-    d.u = (float*)p;
-    CN_HIP(hipMemcpyAsync(e->d_desc, &d, sizeof d, hipMemcpyHostToDevice, e->stream));
+    if ((st = dalloc(e, c.allocs, &p, tot * 4)) != CN_OK) return st;
+    c.dev.u = (float*)p;
+    if ((st = upload_desc(e, 0)) != CN_OK) return st;
   }
   // Recompute the convolution of the state the most recent step started from.
-  StepLaunch sl = base_launch(e, e->cur ^ 1);
-  sl.write_u = true;
-  sl.skip_diffusion = true;
-  CN_HIP(launch_step((int)e->N, sl, e->stream));
-  CN_HIP(hipMemcpyAsync(out, d.u, tot * 4, hipMemcpyDeviceToHost, e->stream));
+  for (const SizeClass& k : e->cls) {
+    StepLaunch sl = class_launch(e, k, e->cur ^ 1, e->cur ^ 1);
+    sl.write_u = true;
+    sl.skip_diffusion = true;
+    CN_HIP(launch_step(k.N, sl, e->stream));
+  }
+  CN_HIP(hipMemcpyAsync(out, c.dev.u, tot * 4, hipMemcpyDeviceToHost, e->stream));
   CN_HIP(hipStreamSynchronize(e->stream));
   const float scale = (float)P;  // unnormalised like the reference's conv buffer (exact: power of 2)
-  for (size_t i = 0; i < tot; ++i) out[i] *= scale;
+  for (size_t j = 0; j < tot; ++j) out[j] *= scale;
   return CN_OK;
 }
 
-cn_status cn_mass_fp64(cn_engine* e, double* out, size_t n_channels) {
+cn_status cn_creature_mass_fp64(cn_engine* e, uint32_t i, double* out, size_t n_channels) {
   clear_error();
-  if (!e || !out) return set_error(CN_ERR_INVALID_ARG, "null argument");
-  if (n_channels != e->C) return set_error(CN_ERR_INVALID_ARG, "expected %u channels", e->C);
+  cn_status st = check_creature(e, i);
+  if (st != CN_OK) return st;
+  if (!out) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  Creature& c = e->cr[i];
+  if (n_channels != c.C) return set_error(CN_ERR_INVALID_ARG, "expected %u channels", c.C);
   CN_HIP(hipSetDevice(e->device));
-  CN_HIP(launch_mass(e->d_desc, 1, e->cur, (int)e->C, e->mass_partial, e->mass_nblk, e->stream));
+  const SizeClass& k = e->cls[c.cls];
+  CN_HIP(launch_mass(k.d_descs, (int)k.members.size(), e->cur, k.max_c, e->mass_partial, e->mass_nblk, e->stream));
   std::vector<double> part((size_t)kMaxChannels * e->mass_nblk * 2);
-  CN_HIP(hipMemcpyAsync(part.data(), e->mass_partial, part.size() * 8, hipMemcpyDeviceToHost, e->stream));
+  CN_HIP(hipMemcpyAsync(part.data(), e->mass_partial + (size_t)c.idx * kMaxChannels * e->mass_nblk,
+                        part.size() * 8, hipMemcpyDeviceToHost, e->stream));
   CN_HIP(hipStreamSynchronize(e->stream));
-  for (uint32_t c = 0; c < e->C; ++c) {
+  for (uint32_t ch = 0; ch < c.C; ++ch) {
     double s = 0.0, comp = 0.0;
     for (int b = 0; b < e->mass_nblk; ++b) {
-      const double* pr = &part[((size_t)c * e->mass_nblk + b) * 2];
+      const double* pr = &part[((size_t)ch * e->mass_nblk + b) * 2];
       for (int j = 0; j < 2; ++j) {
         const double x = pr[j], t = s + x;
         comp += (std::fabs(s) >= std::fabs(x)) ? (s - t) + x : (x - t) + s;
         s = t;
       }
     }
-    out[c] = s + comp;
+    out[ch] = s + comp;
   }
   return CN_OK;
+}
+
+cn_status cn_mass_fp64(cn_engine* e, double* out, size_t n_channels) {
+  return cn_creature_mass_fp64(e, 0, out, n_channels);
 }
 
 cn_status cn_engine_get_info(const cn_engine* e, cn_engine_info* out) {
   clear_error();
   if (!e || !out) return set_error(CN_ERR_INVALID_ARG, "null argument");
-  out->n = e->N;
-  out->num_channels = e->C;
-  out->num_kernels = e->K;
+  out->n = e->cr[0].N;
+  out->num_channels = e->cr[0].C;
+  out->num_kernels = e->cr[0].K;
   out->steps_done = e->steps;
   out->device_bytes = e->bytes;
+  return CN_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// state <-> device pointers (caller-owned device memory; both calls synchronise)
+// ---------------------------------------------------------------------------------------
+cn_status cn_creature_state_from_device(cn_engine* e, uint32_t i, const float* d_src, size_t n) {
+  clear_error();
+  cn_status st = check_creature(e, i);
+  if (st != CN_OK) return st;
+  if (!d_src) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  if (n != e->cr[i].slot_elems()) return set_error(CN_ERR_INVALID_ARG, "expected %zu values", e->cr[i].slot_elems());
+  CN_HIP(hipSetDevice(e->device));
+  CN_HIP(hipMemcpyAsync(cur_state(e, i), d_src, n * 4, hipMemcpyDeviceToDevice, e->stream));
+  CN_HIP(hipStreamSynchronize(e->stream));
+  e->have_prev = false;
+  e->ro_fwd_done = false;
+  return CN_OK;
+}
+
+cn_status cn_creature_state_to_device(cn_engine* e, uint32_t i, float* d_dst, size_t n) {
+  clear_error();
+  cn_status st = check_creature(e, i);
+  if (st != CN_OK) return st;
+  if (!d_dst) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  if (n != e->cr[i].slot_elems()) return set_error(CN_ERR_INVALID_ARG, "expected %zu values", e->cr[i].slot_elems());
+  CN_HIP(hipSetDevice(e->device));
+  CN_HIP(hipMemcpyAsync(d_dst, cur_state(e, i), n * 4, hipMemcpyDeviceToDevice, e->stream));
+  CN_HIP(hipStreamSynchronize(e->stream));
+  return CN_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// rollouts
+// ---------------------------------------------------------------------------------------
+cn_status cn_creature_set_target(cn_engine* e, uint32_t i, const float* target, size_t n) {
+  clear_error();
+  cn_status st = check_creature(e, i);
+  if (st != CN_OK) return st;
+  if (!target) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  Creature& c = e->cr[i];
+  if (n != c.slot_elems())
+    return set_error(CN_ERR_INVALID_ARG, "target has %zu values, expected %zu", n, c.slot_elems());
+  if (!finite_all(target, n)) return set_error(CN_ERR_INVALID_ARG, "target not finite");
+  CN_HIP(hipSetDevice(e->device));
+  if ((st = ensure_adjoint(e)) != CN_OK) return st;
+  CN_HIP(hipMemcpyAsync(c.dev.target, target, n * 4, hipMemcpyHostToDevice, e->stream));
+  CN_HIP(hipStreamSynchronize(e->stream));
+  c.has_target = true;
+  return CN_OK;
+}
+
+cn_status cn_rollout_grad(cn_engine* e, uint32_t steps, uint32_t ckpt_every) {
+  clear_error();
+  if (!e) return set_error(CN_ERR_INVALID_ARG, "null engine");
+  for (size_t i = 0; i < e->cr.size(); ++i)
+    if (!e->cr[i].has_target)
+      return set_error(CN_ERR_STATE, "creature %zu has no target (cn_creature_set_target)", i);
+  CN_HIP(hipSetDevice(e->device));
+  cn_status st;
+  if ((st = normalise_current(e)) != CN_OK) return st;
+  if ((st = prepare_rollout(e, steps, ckpt_every)) != CN_OK) return st;
+  CN_HIP(hipGraphLaunch(e->ro.graph, e->stream));
+  e->ro_fwd_done = true;
+  return CN_OK;
+}
+
+cn_status cn_rollout_forward(cn_engine* e, uint32_t steps, uint32_t ckpt_every) {
+  clear_error();
+  if (!e) return set_error(CN_ERR_INVALID_ARG, "null engine");
+  CN_HIP(hipSetDevice(e->device));
+  cn_status st;
+  if ((st = normalise_current(e)) != CN_OK) return st;
+  if ((st = prepare_rollout(e, steps, ckpt_every)) != CN_OK) return st;
+  CN_HIP(hipGraphLaunch(e->ro.fwd, e->stream));
+  e->ro_fwd_done = true;
+  return CN_OK;
+}
+
+cn_status cn_creature_rollout_final(cn_engine* e, uint32_t i, float* out, size_t n, int device_pointer) {
+  clear_error();
+  cn_status st = check_creature(e, i);
+  if (st != CN_OK) return st;
+  if (!e->ro_fwd_done) return set_error(CN_ERR_STATE, "no rollout forward since the last state change");
+  Creature& c = e->cr[i];
+  if (!out || n != c.slot_elems()) return set_error(CN_ERR_INVALID_ARG, "bad output buffer");
+  CN_HIP(hipSetDevice(e->device));
+  CN_HIP(hipMemcpyAsync(out, c.dev.state + (size_t)e->ro.final_slot * c.slot_elems(), n * 4,
+                        device_pointer ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, e->stream));
+  CN_HIP(hipStreamSynchronize(e->stream));
+  return CN_OK;
+}
+
+cn_status cn_rollout_backward(cn_engine* e, const float* const* d_grad_final, uint32_t count) {
+  clear_error();
+  if (!e || !d_grad_final) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  if (count != e->cr.size()) return set_error(CN_ERR_INVALID_ARG, "expected %zu gradient pointers", e->cr.size());
+  if (!e->ro_fwd_done) return set_error(CN_ERR_STATE, "no rollout forward since the last state change");
+  for (uint32_t i = 0; i < count; ++i)
+    if (!d_grad_final[i]) return set_error(CN_ERR_INVALID_ARG, "null gradient pointer %u", i);
+  CN_HIP(hipSetDevice(e->device));
+  for (uint32_t i = 0; i < count; ++i) {
+    const Creature& c = e->cr[i];
+    CN_HIP(hipMemcpyAsync(c.dev.grad, d_grad_final[i], c.slot_elems() * 4, hipMemcpyDeviceToDevice, e->stream));
+  }
+  const SlotPlan p = plan_slots(e->ro.T, e->ro.ckpt);
+  int gslot = 0;
+  CN_HIP(record_backward(e, p, e->ro.T, e->ro.ckpt, 0, &gslot));
+  CN_HIP(hipStreamSynchronize(e->stream));
+  e->ro.grad_slot = gslot;
+  return CN_OK;
+}
+
+cn_status cn_creature_get_loss(cn_engine* e, uint32_t i, double* loss) {
+  clear_error();
+  cn_status st = check_creature(e, i);
+  if (st != CN_OK) return st;
+  if (!loss) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  Creature& c = e->cr[i];
+  if (!c.adj) return set_error(CN_ERR_STATE, "no rollout has run");
+  CN_HIP(hipSetDevice(e->device));
+  std::vector<float> part(kLossPartials);
+  CN_HIP(hipMemcpyAsync(part.data(), c.dev.lossp, part.size() * 4, hipMemcpyDeviceToHost, e->stream));
+  CN_HIP(hipStreamSynchronize(e->stream));
+  double s = 0.0;
+  for (float v : part) s += v;
+  *loss = s / (double)c.slot_elems();
+  return CN_OK;
+}
+
+cn_status cn_creature_get_grads(cn_engine* e, uint32_t i, float* grad_state0, size_t n, float* grad_mu,
+                                float* grad_sigma, float* grad_w, size_t K, int device_pointers) {
+  clear_error();
+  cn_status st = check_creature(e, i);
+  if (st != CN_OK) return st;
+  Creature& c = e->cr[i];
+  if (!c.adj) return set_error(CN_ERR_STATE, "no rollout backward has run");
+  if (grad_state0 && n != c.slot_elems())
+    return set_error(CN_ERR_INVALID_ARG, "grad_state0 must hold %zu values", c.slot_elems());
+  if ((grad_mu || grad_sigma || grad_w) && K != c.K) return set_error(CN_ERR_INVALID_ARG, "expected K = %u", c.K);
+  CN_HIP(hipSetDevice(e->device));
+  const hipMemcpyKind kind = device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  if (grad_state0)
+    CN_HIP(hipMemcpyAsync(grad_state0, c.dev.grad + (size_t)e->ro.grad_slot * c.slot_elems(), n * 4, kind,
+                          e->stream));
+  float* outs[3] = {grad_mu, grad_sigma, grad_w};
+  for (int j = 0; j < 3; ++j)
+    if (outs[j]) CN_HIP(hipMemcpyAsync(outs[j], c.dev.gparam + (size_t)j * c.K, c.K * 4, kind, e->stream));
+  CN_HIP(hipStreamSynchronize(e->stream));
   return CN_OK;
 }
 

@@ -13,6 +13,9 @@ std::vector<float> make_twiddles();
 cn_status validate_creature(uint32_t n, uint32_t C, uint32_t K, const uint32_t* src, const uint32_t* dst,
                             const float* mu, const float* sigma, const float* w, float temp);
 
+cn_status validate_growth(uint32_t K, const float* mu, const float* sigma, const float* w);
+void fill_growth(CreatureDev& d, uint32_t K, const float* mu, const float* sigma, const float* w);
+
 // Fills the CSR kernel lists and growth parameters of a descriptor.
 void fill_desc_params(CreatureDev& d, uint32_t C, uint32_t K, const uint32_t* src, const uint32_t* dst,
                       const float* mu, const float* sigma, const float* w, float temp);

@@ -8,8 +8,9 @@ namespace cn {
 struct StepLaunch {
   const CreatureDev* descs = nullptr;  // device array of `count` descriptors, all with the same N
   int count = 0;
-  int parity = 0;                      // state[parity] -> state[parity ^ 1]
+  int sin = 0, sout = 1;               // state slot read / written
   int max_c = 0;                       // max channels over the descriptors
+  int max_k = 0;                       // max kernels over the descriptors
   const float2* tw = nullptr;          // kTwN-entry twiddle table
   bool write_u = false;                // also write the real convolution results to desc.u
   bool skip_diffusion = false;         // stop after the affinity (conv readback)
@@ -20,7 +21,16 @@ struct StepLaunch {
 };
 
 hipError_t launch_step(int N, const StepLaunch& sl, hipStream_t st);
-hipError_t launch_mass(const CreatureDev* descs, int count, int parity, int max_c, double2* partial, int nblk,
+// One adjoint step of state slot sl.sin: gradient slot gin (dL/dstate after the step) ->
+// gout (dL/dstate before it); parameter gradients accumulate into desc.gparam.
+hipError_t launch_adjoint(int N, const StepLaunch& sl, int gin, int gout, hipStream_t st);
+// dL/dx = 2 (x - target) / numel into gradient slot gout, loss partials into desc.lossp.
+hipError_t launch_mse_grad(const CreatureDev* descs, int count, int sfin, int gout, hipStream_t st);
+hipError_t launch_zero_gparam(const CreatureDev* descs, int count, hipStream_t st);
+int row_blocks(int N);  // row-transform blocks per plane (parameter-gradient partial count)
+constexpr int kLossPartials = 256;
+
+hipError_t launch_mass(const CreatureDev* descs, int count, int slot, int max_c, double2* partial, int nblk,
                        hipStream_t st);
 
 }  // namespace cn

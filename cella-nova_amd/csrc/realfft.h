@@ -1,0 +1,63 @@
+// Real-row transforms built on the block FFT (fft.h): an N-point real row is handled as an
+// N/2-point complex transform of z[n] = x[2n] + i x[2n+1] plus a split/merge step.  The half
+// spectrum of a row is stored as N/2 complex values with slot 0 packing the two real bins
+// (X[0], X[N/2]) as (re, im).
+#pragma once
+#include "fft.h"
+#include "types.h"
+
+namespace cn {
+
+// After the forward L-point transform of z (L = N/2), v holds Z at out_pos positions; turn it
+// into the real row's half spectrum and store it to `out` (L complex).  Uses `lds` (the same
+// buffer as the transform) for the Z[L-k] partner exchange.
+template <int N, int E, int ROWS>
+__device__ __forceinline__ void r2c_finish(const float2* v, float2* lds, int t, int lane,
+                                           const float2* __restrict__ tw, float2* out, bool store) {
+  constexpr int L = N / 2;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < E; ++i) lds[LdsMap<L, ROWS, false>::at(out_pos<L, E, false>(t, i), lane)] = v[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int p = out_pos<L, E, false>(t, i);
+    const float2 zp = v[i];
+    float2 X;
+    if (p == 0) {
+      X = make_float2(zp.x + zp.y, zp.x - zp.y);
+    } else {
+      const float2 zq = lds[LdsMap<L, ROWS, false>::at(L - p, lane)];
+      const float2 ze = make_float2(0.5f * (zp.x + zq.x), 0.5f * (zp.y - zq.y));
+      const float2 dd = make_float2(zp.x - zq.x, zp.y + zq.y);  // zp - conj(zq)
+      const float2 zo = make_float2(0.5f * dd.y, -0.5f * dd.x);  // dd / (2i)
+      X = c_add(ze, c_mul(tw[p * (kTwN / N)], zo));
+    }
+    if (store) out[p] = X;
+  }
+}
+
+// Loads a half-spectrum row X (L = N/2 complex) and forms the L-point input Z2 of the inverse
+// transform whose unnormalised output is z[n] = x[2n] + i x[2n+1] with x the unnormalised
+// N-point real inverse of X.
+template <int N, int E>
+__device__ __forceinline__ void c2r_start(const float2* __restrict__ X, int t, const float2* __restrict__ tw,
+                                          float2* v) {
+  constexpr int L = N / 2;
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int p = in_pos<L, E, true>(t, i);
+    const float2 xp = X[p];
+    if (p == 0) {
+      v[i] = make_float2(xp.x + xp.y, xp.x - xp.y);
+    } else {
+      const float2 xq = X[L - p];
+      const float2 e2 = make_float2(xp.x + xq.x, xp.y - xq.y);  // xp + conj(xq)
+      const float2 dd = make_float2(xp.x - xq.x, xp.y + xq.y);  // xp - conj(xq)
+      const float2 o2 = c_mulc(dd, tw[p * (kTwN / N)]);          // * W_N^{-p}
+      v[i] = make_float2(e2.x - o2.y, e2.y + o2.x);             // e2 + i*o2
+    }
+  }
+}
+
+}  // namespace cn

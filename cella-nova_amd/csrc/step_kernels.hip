@@ -11,32 +11,20 @@
 //   k_diffusion       halo-tiled 3x3 Z sum and mass-conserving gather (compute_1024.wgsl:341-413)
 // The copy phase is a pointer swap (ping-pong state).
 #include <hip/hip_runtime.h>
+#include "cfg.h"
 #include "fft.h"
-#include "types.h"
 #include "launch.h"
+#include "realfft.h"
+#include "types.h"
 
 namespace cn {
-
-template <int N>
-struct Cfg {
-  static constexpr int L_ROW = N / 2;
-  static constexpr int E_ROW = L_ROW >= 16 ? 16 : L_ROW;
-  static constexpr int T_ROW = L_ROW / E_ROW;
-  static constexpr int ROWS = (256 / T_ROW) < N ? (256 / T_ROW) : N;
-  static constexpr int ROW_THREADS = ROWS * T_ROW;
-  static constexpr int E_COL = N >= 16 ? 16 : N;
-  static constexpr int T_COL = N / E_COL;
-  // column tile width (complex columns per block of the fused column pass)
-  static constexpr int TC_DEF = N >= 2048 ? 4 : (N >= 1024 ? 8 : 16);
-  static constexpr int TC = TC_DEF < N / 2 ? TC_DEF : N / 2;
-};
 
 // ---------------------------------------------------------------------------------------
 // Row forward: state rows (real, N) -> half spectrum rows (N/2 complex, slot 0 = (X0, X_{N/2}))
 // ---------------------------------------------------------------------------------------
 template <int N>
 __global__ void __launch_bounds__(Cfg<N>::ROW_THREADS)
-k_row_fwd(const CreatureDev* __restrict__ descs, int parity, const float2* __restrict__ tw) {
+k_row_fwd(const CreatureDev* __restrict__ descs, int sin, const float2* __restrict__ tw) {
   using CF = Cfg<N>;
   constexpr int L = CF::L_ROW, E = CF::E_ROW, T = CF::T_ROW, ROWS = CF::ROWS;
   const CreatureDev& d = descs[blockIdx.z];
@@ -49,7 +37,7 @@ k_row_fwd(const CreatureDev* __restrict__ descs, int parity, const float2* __res
   const bool valid = row < nrows;
   if (!valid) row = nrows - 1;
 
-  const float2* in = reinterpret_cast<const float2*>(d.state[parity] + (size_t)row * N);
+  const float2* in = reinterpret_cast<const float2*>(slot_ptr(d, sin) + (size_t)row * N);
   float2 v[E];
 #pragma unroll
   for (int i = 0; i < E; ++i) v[i] = in[in_pos<L, E, false>(t, i)];
@@ -58,27 +46,7 @@ k_row_fwd(const CreatureDev* __restrict__ descs, int parity, const float2* __res
   fft_run<L, E, ROWS, false, false, kTwN>(v, lds, t, lane, tw);
 
   // Split the packed transform Z of z[n] = x[2n] + i x[2n+1] into the real row's spectrum.
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < E; ++i) lds[LdsMap<L, ROWS, false>::at(out_pos<L, E, false>(t, i), lane)] = v[i];
-  __syncthreads();
-  float2* out = reinterpret_cast<float2*>(d.spec) + (size_t)row * L;
-#pragma unroll
-  for (int i = 0; i < E; ++i) {
-    const int p = out_pos<L, E, false>(t, i);
-    const float2 zp = v[i];
-    float2 X;
-    if (p == 0) {
-      X = make_float2(zp.x + zp.y, zp.x - zp.y);
-    } else {
-      const float2 zq = lds[LdsMap<L, ROWS, false>::at(L - p, lane)];
-      const float2 ze = make_float2(0.5f * (zp.x + zq.x), 0.5f * (zp.y - zq.y));
-      const float2 dd = make_float2(zp.x - zq.x, zp.y + zq.y);  // zp - conj(zq)
-      const float2 zo = make_float2(0.5f * dd.y, -0.5f * dd.x);  // dd / (2i)
-      X = c_add(ze, c_mul(tw[p * (kTwN / N)], zo));
-    }
-    if (valid) out[p] = X;
-  }
+  r2c_finish<N, E, ROWS>(v, lds, t, lane, tw, reinterpret_cast<float2*>(d.spec) + (size_t)row * L, valid);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -96,13 +64,7 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
   if (c >= d.C) return;
   const int kb = d.src_off[c], ke = d.src_off[c + 1];
   if (kb == ke) return;
-  // Workgroups are dealt round-robin to the 8 XCDs (separate L2s).  Give each XCD a
-  // contiguous run of tiles so that neighbouring tiles, which share 128-byte lines of every
-  // row, are fetched through the same L2.
-  constexpr int NTILES = (N / 2) / TC;
-  const int bx = blockIdx.x;
-  const int tile = (NTILES % 8 == 0) ? (bx % 8) * (NTILES / 8) + bx / 8 : bx;
-  const int x0 = tile * TC;
+  const int x0 = xcd_tile<(N / 2) / TC>(blockIdx.x) * TC;
   const int lane = threadIdx.x % TC;
   int t = threadIdx.x / TC;
   const int x = x0 + lane;
@@ -201,20 +163,7 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
       const float mu = d.mu[k], is2 = d.inv_s2[k], wk = d.w[k];
       const float2* X = reinterpret_cast<const float2*>(d.uhat) + ((size_t)k * N + y) * L;
       float2 v[E];
-#pragma unroll
-      for (int i = 0; i < E; ++i) {
-        const int p = in_pos<L, E, true>(t, i);
-        const float2 xp = X[p];
-        if (p == 0) {
-          v[i] = make_float2(xp.x + xp.y, xp.x - xp.y);
-        } else {
-          const float2 xq = X[L - p];
-          const float2 e2 = make_float2(xp.x + xq.x, xp.y - xq.y);       // xp + conj(xq)
-          const float2 dd = make_float2(xp.x - xq.x, xp.y + xq.y);       // xp - conj(xq)
-          const float2 o2 = c_mulc(dd, tw[p * (kTwN / N)]);               // * W_N^{-p}
-          v[i] = make_float2(e2.x - o2.y, e2.y + o2.x);                  // e2 + i*o2
-        }
-      }
+      c2r_start<N, E>(X, t, tw, v);
       fft_run<L, E, ROWS, false, true, kTwN>(v, lds, t, lane, tw);
 #pragma unroll
       for (int i = 0; i < E; ++i) {
@@ -260,7 +209,7 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
 }
 
 __global__ void __launch_bounds__(kDiffThreads)
-k_diffusion(const CreatureDev* __restrict__ descs, int parity, int N) {
+k_diffusion(const CreatureDev* __restrict__ descs, int sin, int sout, int N) {
   constexpr int TW = kDiffTW, TH = kDiffTH;
   const CreatureDev& d = descs[blockIdx.z];
   if (d.N != N) return;
@@ -272,8 +221,8 @@ k_diffusion(const CreatureDev* __restrict__ descs, int parity, int N) {
   const int mask = N - 1;
   const size_t plane = (size_t)N * N;
   const float* A = d.A + c * plane;
-  const float* s = d.state[parity] + c * plane;
-  float* o = d.state[parity ^ 1] + c * plane;
+  const float* s = slot_ptr(d, sin) + c * plane;
+  float* o = slot_ptr(d, sout) + c * plane;
 
   __shared__ float sA[TH + 4][TW + 5];
   __shared__ float sQ[TH + 2][TW + 3];
@@ -333,13 +282,13 @@ __device__ __forceinline__ void neumaier_add(double& s, double& comp, double x) 
 constexpr int kMassThreads = 256;
 
 __global__ void __launch_bounds__(kMassThreads)
-k_mass(const CreatureDev* __restrict__ descs, int parity, double2* __restrict__ partial, int nblk) {
+k_mass(const CreatureDev* __restrict__ descs, int slot, double2* __restrict__ partial, int nblk) {
   const CreatureDev& d = descs[blockIdx.z];
   const int c = blockIdx.y;
   double2* dst = partial + ((size_t)blockIdx.z * kMaxChannels + c) * nblk + blockIdx.x;
   if (c >= d.C) return;
   const size_t plane = (size_t)d.N * d.N;
-  const float* s = d.state[parity] + c * plane;
+  const float* s = slot_ptr(d, slot) + c * plane;
   double sum = 0.0, comp = 0.0;
   for (size_t i = (size_t)blockIdx.x * kMassThreads + threadIdx.x; i < plane; i += (size_t)nblk * kMassThreads)
     neumaier_add(sum, comp, (double)s[i]);
@@ -366,7 +315,7 @@ static hipError_t record(hipEvent_t ev, hipStream_t st) { return ev ? hipEventRe
 
 template <int N, int TC>
 static hipError_t launch_col(const StepLaunch& sl, hipStream_t st) {
-  if constexpr (TC > N / 2 || (size_t)N * TC * 8 + (size_t)N * 8 > 160 * 1024) {
+  if constexpr (!col_tc_ok<N, TC>()) {
     return hipErrorInvalidValue;
   } else {
     const dim3 g_col((N / 2) / TC, sl.max_c, sl.count);
@@ -381,7 +330,7 @@ static hipError_t launch_step_n(const StepLaunch& sl, hipStream_t st) {
   hipError_t e;
   if ((e = record(sl.ev[0], st)) != hipSuccess) return e;
   const dim3 g_row((sl.max_c * N + CF::ROWS - 1) / CF::ROWS, 1, sl.count);
-  k_row_fwd<N><<<g_row, CF::ROW_THREADS, 0, st>>>(sl.descs, sl.parity, sl.tw);
+  k_row_fwd<N><<<g_row, CF::ROW_THREADS, 0, st>>>(sl.descs, sl.sin, sl.tw);
   if ((e = record(sl.ev[1], st)) != hipSuccess) return e;
   switch (sl.col_tc ? sl.col_tc : CF::TC) {
     case 2: e = launch_col<N, 2>(sl, st); break;
@@ -401,7 +350,7 @@ static hipError_t launch_step_n(const StepLaunch& sl, hipStream_t st) {
   if (!sl.skip_diffusion) {
     const int tiles_x = (N + kDiffTW - 1) / kDiffTW, tiles_y = (N + kDiffTH - 1) / kDiffTH;
     const dim3 g_dif(tiles_x, tiles_y * sl.max_c, sl.count);
-    k_diffusion<<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.parity, N);
+    k_diffusion<<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N);
   }
   if ((e = record(sl.ev[4], st)) != hipSuccess) return e;
   return hipGetLastError();
@@ -421,9 +370,9 @@ hipError_t launch_step(int N, const StepLaunch& sl, hipStream_t st) {
   }
 }
 
-hipError_t launch_mass(const CreatureDev* descs, int count, int parity, int max_c, double2* partial, int nblk,
+hipError_t launch_mass(const CreatureDev* descs, int count, int slot, int max_c, double2* partial, int nblk,
                        hipStream_t st) {
-  k_mass<<<dim3(nblk, max_c, count), kMassThreads, 0, st>>>(descs, parity, partial, nblk);
+  k_mass<<<dim3(nblk, max_c, count), kMassThreads, 0, st>>>(descs, slot, partial, nblk);
   return hipGetLastError();
 }
 

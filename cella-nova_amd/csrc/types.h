@@ -1,16 +1,26 @@
 // Shared host/device layout of one creature (one DiffusionLenia world) on the device.
 //
-// HBM layout (all planes row-major, channel/kernel-major, 256-byte aligned bases):
-//   state[2]  [C][N][N]    f32   ping-pong world state (the reference's channel/new_channel)
-//   spec      [C][N][N/2]  c64   row-R2C spectra of the state, then column-transformed in place
-//                                by nobody (the fused column pass writes uhat instead)
-//   khalf     [K][N][N/2]  c64   Hermitian part of each kernel spectrum, pre-scaled by 1/N^2,
-//                                columns kx = 0 .. N/2-1
-//   knyq      [K][N]       c64   the kx = N/2 (Nyquist) column of the same spectra
-//   uhat      [K][N][N/2]  c64   row spectra of every convolution result (after the inverse
-//                                column pass); slot kx = 0 packs (DC, Nyquist) as (re, im)
-//   A         [C][N][N]    f32   affinity exp(temp * sum_k w_k G_k) per destination channel
-//   u         [K][N][N]    f32   optional: convolution results (only for readback/adjoints)
+// HBM layout (all planes row-major, channel/kernel-major):
+//   state   [nslots][C][N][N] f32  world-state slots: 0/1 ping-pong current state (the
+//                                  reference's channel/new_channel), the rest are rollout
+//                                  checkpoints and recompute buffers.  Kernels take slot indices.
+//   spec    [C][N][N/2]  c64   row-R2C spectra of a state (also reused by the adjoint)
+//   khalf   [K][N][N/2]  c64   Hermitian part of each kernel spectrum, pre-scaled by 1/N^2,
+//                              columns kx = 0 .. N/2-1
+//   knyq    [K][N]       c64   the kx = N/2 (Nyquist) column of the same spectra
+//   uhat    [K][N][N/2]  c64   row spectra of every convolution result (after the inverse
+//                              column pass); slot kx = 0 packs (DC, Nyquist) as (re, im)
+//   A       [C][N][N]    f32   affinity exp(temp * sum_k w_k G_k) per destination channel
+//   u       [K][N][N]    f32   convolution results (readback and adjoint only)
+// Adjoint buffers (allocated on first rollout):
+//   grad    [2][C][N][N] f32   ping-pong dL/dstate
+//   gaff    [C][N][N]    f32   dL/d(pre-exp affinity)
+//   dsd     [C][N][N]    f32   diffusion part of dL/dstate
+//   specg   [K][N][N/2]  c64   row spectra of dL/du_k
+//   partial [nblk][K][3] f32   per-block sums for dL/dmu, dL/dsigma, dL/dw
+//   gparam  [2][3][K]    f32   accumulated (mu, sigma, w) gradients and their Kahan terms
+//   target  [C][N][N]    f32   loss target
+//   lossp   [nblk]       f32   per-block loss partial sums
 #pragma once
 #include <stdint.h>
 
@@ -21,13 +31,22 @@ constexpr int kMaxKernels = 256;
 constexpr int kTwN = 4096;         // twiddle table length; supports N <= 4096
 
 struct CreatureDev {
-  float* state[2];
+  float* state;
+  int nslots;
   void* spec;          // float2*
   const void* khalf;   // float2*
   const void* knyq;    // float2*
   void* uhat;          // float2*
   float* A;
-  float* u;            // may be null
+  float* u;
+  float* grad;
+  float* gaff;
+  float* dsd;
+  void* specg;         // float2*
+  float* partial;
+  float* gparam;
+  float* target;
+  float* lossp;
   int N, C, K;
   float temp;
   // kernels grouped by source channel (CSR): src_k[src_off[c] .. src_off[c+1]) in increasing k
@@ -36,9 +55,18 @@ struct CreatureDev {
   // kernels grouped by destination channel (CSR), increasing k inside a group
   int dst_off[kMaxChannels + 1];
   int dst_k[kMaxKernels];
+  int src_of[kMaxKernels];     // src[k]
+  int dst_of[kMaxKernels];     // dst[k]
   float mu[kMaxKernels];
+  float sigma[kMaxKernels];
   float inv_s2[kMaxKernels];   // 1 / sigma^2
   float w[kMaxKernels];
 };
+
+#ifdef __HIPCC__
+__device__ __forceinline__ float* slot_ptr(const CreatureDev& d, int s) {
+  return d.state + (size_t)s * d.C * d.N * d.N;
+}
+#endif
 
 }  // namespace cn

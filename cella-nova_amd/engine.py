@@ -1,6 +1,6 @@
 """Python mirror of the reference orchestrator `GpuMaceLenia` (ml-rs/orchestrator/mod.rs:45-541)
-over the C ABI.  Same method set and argument meaning; errors raise CellaNovaError instead of
-panicking (mod.rs:94,208,218)."""
+over the C ABI, plus the batch and rollout interfaces.  Same method set and argument meaning
+as the reference; errors raise CellaNovaError instead of panicking (mod.rs:94,208,218)."""
 import ctypes
 
 import numpy as np
@@ -16,38 +16,44 @@ def _f32(a):
     return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
 
 
-class GpuMaceLenia:
-    """GpuMaceLenia::new (mod.rs:79-198): square grid shape[0] x shape[0], C channels, K kernels,
-    c0 = input channel of each kernel, c1 = output channel, growth mu/sigma/weights, temp."""
+def _desc(n, C, K, src, dst, mu, sigma, w, temp, device, keep):
+    arrs = [_u32(src), _u32(dst), _f32(mu), _f32(sigma), _f32(w)]
+    for a in arrs:
+        if a.shape != (K,):
+            raise ValueError(f"per-kernel arrays must have {K} entries")
+    keep.extend(arrs)
+    P = ctypes.POINTER
+    return EngineDesc(int(n), int(C), int(K), arrs[0].ctypes.data_as(P(ctypes.c_uint32)),
+                      arrs[1].ctypes.data_as(P(ctypes.c_uint32)), arrs[2].ctypes.data_as(P(ctypes.c_float)),
+                      arrs[3].ctypes.data_as(P(ctypes.c_float)), arrs[4].ctypes.data_as(P(ctypes.c_float)),
+                      float(temp), int(device))
 
-    def __init__(self, shape, num_channels, num_kernels, c0, c1, mu, sigma, weights, temp=1.0, device=0):
-        if isinstance(shape, int):
-            shape = (shape, shape)
-        if len(shape) != 2:
-            raise ValueError("DiffusionLenia requires 2D grids")  # mod.rs:94
-        n = int(shape[0])
-        self._keep = [_u32(c0), _u32(c1), _f32(mu), _f32(sigma), _f32(weights)]
-        for a in self._keep:
-            if a.shape != (num_kernels,):
-                raise ValueError(f"per-kernel arrays must have {num_kernels} entries")
-        src, dst, m, s, w = self._keep
-        desc = EngineDesc(
-            n, num_channels, num_kernels,
-            src.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
-            dst.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
-            m.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
-            s.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
-            w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
-            float(temp), int(device))
+
+def _ptr(x):
+    """Device address of a torch tensor, or host address of a numpy array."""
+    if hasattr(x, "data_ptr"):
+        return x.data_ptr()
+    return x.ctypes.data
+
+
+class _Engine:
+    """Creature-indexed operations shared by GpuMaceLenia (one creature) and CreatureBatch."""
+
+    _h = None
+
+    def _init(self, descs, device):
+        self._shapes = [(int(d.num_channels), int(d.n), int(d.num_kernels)) for d in descs]
         h = ctypes.c_void_p()
-        check(lib().cn_engine_create(ctypes.byref(desc), ctypes.byref(h)))
+        if len(descs) == 1:
+            check(lib().cn_engine_create(ctypes.byref(descs[0]), ctypes.byref(h)))
+        else:
+            arr = (EngineDesc * len(descs))(*descs)
+            check(lib().cn_batch_create(arr, len(descs), int(device), ctypes.byref(h)))
         self._h = h
-        self.n, self.num_channels, self.num_kernels = n, int(num_channels), int(num_kernels)
-        self.total_elements = n * n
 
     # --- lifecycle -------------------------------------------------------------------------
     def close(self):
-        if getattr(self, "_h", None) is not None and self._h.value:
+        if self._h is not None and self._h.value:
             check(lib().cn_engine_destroy(self._h))
             self._h = None
 
@@ -63,28 +69,77 @@ class GpuMaceLenia:
     def __exit__(self, *exc):
         self.close()
 
-    # --- uploads (mod.rs:207-226) ---------------------------------------------------------
-    def set_kernel(self, kernel_fft, kernel_idx):
-        """Full N*N complex64 spectrum of kernel `kernel_idx` (complex array or interleaved f32)."""
+    def _elems(self, i):
+        C, n, _ = self._shapes[i]
+        return C * n * n
+
+    # --- per creature ------------------------------------------------------------------------
+    def creature_set_kernel(self, i, k, kernel_fft):
         a = np.asarray(kernel_fft)
         if np.iscomplexobj(a):
             a = np.ascontiguousarray(a.astype(np.complex64)).view(np.float32)
         else:
             a = _f32(a)
         a = a.reshape(-1)
-        check(lib().cn_set_kernel(self._h, int(kernel_idx), a.ctypes.data, a.size // 2))
+        check(lib().cn_creature_set_kernel(self._h, int(i), int(k), a.ctypes.data, a.size // 2))
 
-    def upload_channel(self, data, channel):
-        """f64 values cast to f32 on upload (mod.rs:220); f32 input is uploaded as is."""
-        a = np.asarray(data)
-        if a.dtype == np.float32:
-            a = np.ascontiguousarray(a).reshape(-1)
-            check(lib().cn_upload_channel_f32(self._h, int(channel), a.ctypes.data, a.size))
-        else:
-            a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
-            check(lib().cn_upload_channel_f64(self._h, int(channel), a.ctypes.data, a.size))
+    def creature_upload_channel(self, i, c, data):
+        a = np.ascontiguousarray(np.asarray(data, dtype=np.float32)).reshape(-1)
+        check(lib().cn_creature_upload_channel_f32(self._h, int(i), int(c), a.ctypes.data, a.size))
 
-    # --- stepping (mod.rs:229-233, 348-386, 469-540) --------------------------------------
+    def creature_download(self, i):
+        out = np.empty(self._elems(i), dtype=np.float32)
+        check(lib().cn_creature_download_channels(self._h, int(i), out.ctypes.data, out.size))
+        return out
+
+    def creature_mass(self, i):
+        out = np.empty(self._shapes[i][0], dtype=np.float64)
+        check(lib().cn_creature_mass_fp64(self._h, int(i), out.ctypes.data, out.size))
+        return out
+
+    def creature_set_growth(self, i, mu, sigma, w):
+        m, s, ww = _f32(mu), _f32(sigma), _f32(w)
+        check(lib().cn_creature_set_growth(self._h, int(i), m.ctypes.data, s.ctypes.data, ww.ctypes.data, m.size))
+
+    def creature_state_from_device(self, i, tensor):
+        """Copy a contiguous f32 device tensor (C, N, N) into the creature's current state."""
+        check(lib().cn_creature_state_from_device(self._h, int(i), tensor.data_ptr(), tensor.numel()))
+
+    def creature_state_to_device(self, i, tensor):
+        check(lib().cn_creature_state_to_device(self._h, int(i), tensor.data_ptr(), tensor.numel()))
+
+    def creature_set_target(self, i, target):
+        a = _f32(target).reshape(-1)
+        check(lib().cn_creature_set_target(self._h, int(i), a.ctypes.data, a.size))
+
+    def creature_rollout_final(self, i, out=None):
+        """Final state of the last rollout: into `out` (device tensor) or a new numpy array."""
+        if out is None:
+            arr = np.empty(self._elems(i), dtype=np.float32)
+            check(lib().cn_creature_rollout_final(self._h, int(i), arr.ctypes.data, arr.size, 0))
+            return arr
+        check(lib().cn_creature_rollout_final(self._h, int(i), out.data_ptr(), out.numel(), 1))
+        return out
+
+    def creature_loss(self, i):
+        v = ctypes.c_double()
+        check(lib().cn_creature_get_loss(self._h, int(i), ctypes.byref(v)))
+        return v.value
+
+    def creature_grads(self, i, state=None, mu=None, sigma=None, w=None, device=False):
+        """Gradients of the last backward.  Host: returns numpy arrays.  device=True: fills the
+        given device tensors."""
+        C, n, K = self._shapes[i]
+        if not device:
+            state = np.empty(C * n * n, np.float32)
+            mu, sigma, w = (np.empty(K, np.float32) for _ in range(3))
+        check(lib().cn_creature_get_grads(
+            self._h, int(i), _ptr(state) if state is not None else None, C * n * n,
+            _ptr(mu) if mu is not None else None, _ptr(sigma) if sigma is not None else None,
+            _ptr(w) if w is not None else None, K, 1 if device else 0))
+        return {"state0": state, "mu": mu, "sigma": sigma, "weights": w}
+
+    # --- all creatures -------------------------------------------------------------------------
     def iterate(self):
         check(lib().cn_iterate(self._h))
 
@@ -94,24 +149,69 @@ class GpuMaceLenia:
     def sync(self):
         check(lib().cn_sync(self._h))
 
+    def rollout_grad(self, steps, ckpt_every):
+        """Forward `steps` steps, MSE vs each creature's target, checkpointed adjoint (async)."""
+        check(lib().cn_rollout_grad(self._h, int(steps), int(ckpt_every)))
+
+    def rollout_forward(self, steps, ckpt_every):
+        check(lib().cn_rollout_forward(self._h, int(steps), int(ckpt_every)))
+
+    def rollout_backward(self, grad_finals):
+        """dL/dx_T per creature as contiguous f32 device tensors."""
+        arr = (ctypes.c_void_p * len(grad_finals))(*[g.data_ptr() for g in grad_finals])
+        check(lib().cn_rollout_backward(self._h, arr, len(grad_finals)))
+
+    def profile_kernels(self, steps):
+        """Average device ms per launch over `steps` directly launched steps (first size class):
+        [row forward, fused column, inverse row + growth, diffusion]."""
+        out = np.empty(4, dtype=np.float64)
+        check(lib().cn_profile_kernels(self._h, int(steps), out.ctypes.data, out.size))
+        return out
+
+
+class GpuMaceLenia(_Engine):
+    """GpuMaceLenia::new (mod.rs:79-198): square grid shape[0] x shape[0], C channels, K kernels,
+    c0 = input channel of each kernel, c1 = output channel, growth mu/sigma/weights, temp."""
+
+    def __init__(self, shape, num_channels, num_kernels, c0, c1, mu, sigma, weights, temp=1.0, device=0):
+        if isinstance(shape, int):
+            shape = (shape, shape)
+        if len(shape) != 2:
+            raise ValueError("DiffusionLenia requires 2D grids")  # mod.rs:94
+        n = int(shape[0])
+        self._keep = []
+        self._init([_desc(n, num_channels, num_kernels, c0, c1, mu, sigma, weights, temp, device, self._keep)],
+                   device)
+        self.n, self.num_channels, self.num_kernels = n, int(num_channels), int(num_kernels)
+        self.total_elements = n * n
+
+    # --- uploads (mod.rs:207-226) ---------------------------------------------------------
+    def set_kernel(self, kernel_fft, kernel_idx):
+        """Full N*N complex64 spectrum of kernel `kernel_idx` (complex array or interleaved f32)."""
+        self.creature_set_kernel(0, kernel_idx, kernel_fft)
+
+    def upload_channel(self, data, channel):
+        """f64 values cast to f32 on upload (mod.rs:220); f32 input is uploaded as is."""
+        a = np.asarray(data)
+        if a.dtype == np.float32:
+            self.creature_upload_channel(0, channel, a)
+        else:
+            a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+            check(lib().cn_upload_channel_f64(self._h, int(channel), a.ctypes.data, a.size))
+
+    def set_growth(self, mu, sigma, weights):
+        self.creature_set_growth(0, mu, sigma, weights)
+
+    # --- stepping (mod.rs:229-233, 348-386, 469-540) --------------------------------------
     def timed_iterate(self):
         """[conv, growth, diffusion, copy] device milliseconds of one step."""
         ms = (ctypes.c_double * 4)()
         check(lib().cn_timed_iterate(self._h, ms))
         return [ms[i] for i in range(4)]
 
-    def profile_kernels(self, steps):
-        """Average device ms per launch over `steps` directly launched steps:
-        [row forward, fused column, inverse row + growth, diffusion]."""
-        out = np.empty(4, dtype=np.float64)
-        check(lib().cn_profile_kernels(self._h, int(steps), out.ctypes.data, out.size))
-        return out
-
     # --- downloads (mod.rs:236-299) -------------------------------------------------------
     def download_all_channels(self):
-        out = np.empty(self.num_channels * self.total_elements, dtype=np.float32)
-        check(lib().cn_download_channels(self._h, out.ctypes.data, out.size))
-        return out
+        return self.creature_download(0)
 
     def download_conv_buffer(self):
         """Real part of the unnormalised convolution buffer, K*N*N f32 (mod.rs:267-299)."""
@@ -121,12 +221,29 @@ class GpuMaceLenia:
 
     def mass_fp64(self):
         """Per-channel FP64 compensated mass of the current state."""
-        out = np.empty(self.num_channels, dtype=np.float64)
-        check(lib().cn_mass_fp64(self._h, out.ctypes.data, out.size))
-        return out
+        return self.creature_mass(0)
 
     def info(self):
         i = EngineInfo()
         check(lib().cn_engine_get_info(self._h, ctypes.byref(i)))
         return {"n": i.n, "num_channels": i.num_channels, "num_kernels": i.num_kernels,
                 "steps_done": i.steps_done, "device_bytes": i.device_bytes}
+
+
+class CreatureBatch(_Engine):
+    """Many creatures (mixed N, C, K) advanced together: one launch per kernel per grid size,
+    all captured in one HIP graph per step / per rollout."""
+
+    def __init__(self, creatures, device=0):
+        self._keep = []
+        descs = [_desc(c["n"], c["num_channels"], c["num_kernels"], c["src"], c["dst"], c["mu"], c["sigma"],
+                       c["weights"], c.get("temp", 1.0), device, self._keep) for c in creatures]
+        self._init(descs, device)
+        self.count = len(creatures)
+
+    def load(self, i, cr):
+        """Upload a generated creature's kernel spectra and seed (creatures.generate_creature)."""
+        for k in range(cr["num_kernels"]):
+            self.creature_set_kernel(i, k, cr["kernels_fft"][k])
+        for c in range(cr["num_channels"]):
+            self.creature_upload_channel(i, c, cr["state"][c])

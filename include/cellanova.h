@@ -109,6 +109,58 @@ cn_status cn_engine_get_info(const cn_engine* e, cn_engine_info* out);
 const char* cn_last_error(void);
 
 /* ------------------------------------------------------------------------------------ */
+/* Batches (no reference counterpart: many creatures of mixed N, C, K in one engine)      */
+/* ------------------------------------------------------------------------------------ */
+
+/* An engine holding `count` creatures (descs[i].device is ignored; `device` is used).  Every
+ * step / rollout call advances all creatures together; creatures of equal N share one launch
+ * per kernel (one descriptor table per size class) and all classes are captured in one HIP
+ * graph.  The single-creature calls above act on creature 0. */
+cn_status cn_batch_create(const cn_engine_desc* descs, uint32_t count, int device, cn_engine** out);
+/* Per-creature forms of set_kernel / upload_channel / download_all_channels / mass. */
+cn_status cn_creature_set_kernel(cn_engine* e, uint32_t creature, uint32_t k, const float* interleaved,
+                                 size_t n_complex);
+cn_status cn_creature_upload_channel_f32(cn_engine* e, uint32_t creature, uint32_t channel, const float* data,
+                                         size_t n);
+cn_status cn_creature_download_channels(cn_engine* e, uint32_t creature, float* out, size_t n);
+cn_status cn_creature_mass_fp64(cn_engine* e, uint32_t creature, double* out, size_t n_channels);
+/* Replace the growth parameters (mu, sigma > 0, weights; K each) of one creature. */
+cn_status cn_creature_set_growth(cn_engine* e, uint32_t creature, const float* mu, const float* sigma,
+                                 const float* weights, size_t K);
+/* Copy the current state of a creature from / to caller-owned DEVICE memory (C*N*N f32).
+ * Both synchronise the engine stream; the caller orders its own streams before the call. */
+cn_status cn_creature_state_from_device(cn_engine* e, uint32_t creature, const float* d_src, size_t n);
+cn_status cn_creature_state_to_device(cn_engine* e, uint32_t creature, float* d_dst, size_t n);
+
+/* ------------------------------------------------------------------------------------ */
+/* Differentiable rollouts (no reference counterpart; the reference differentiates its    */
+/* PyTorch model, train/diff_lenia_org.py:27-53, with autograd)                           */
+/* ------------------------------------------------------------------------------------ */
+
+/* Loss target of a creature (C*N*N f32, host memory) for cn_rollout_grad. */
+cn_status cn_creature_set_target(cn_engine* e, uint32_t creature, const float* target, size_t n);
+/* Rollout of `steps` steps of every creature from its current state, keeping a checkpoint
+ * every `ckpt_every` steps, then the MSE loss mean((x_T - target)^2) and the checkpointed
+ * adjoint (recomputing each segment from its checkpoint): dL/dstate0 and dL/d(mu, sigma,
+ * weights) summed over the steps.  The current state is not modified.  Asynchronous; one HIP
+ * graph per (steps, ckpt_every) is captured on first use and replayed afterwards. */
+cn_status cn_rollout_grad(cn_engine* e, uint32_t steps, uint32_t ckpt_every);
+/* Forward half only (keeps the checkpoints for cn_rollout_backward).  Asynchronous. */
+cn_status cn_rollout_forward(cn_engine* e, uint32_t steps, uint32_t ckpt_every);
+/* Final state of the last rollout forward (host memory, or device memory if device_pointer). */
+cn_status cn_creature_rollout_final(cn_engine* e, uint32_t creature, float* out, size_t n, int device_pointer);
+/* Adjoint of the last cn_rollout_forward given dL/dx_T per creature (DEVICE pointers, one per
+ * creature, C*N*N f32 each).  Synchronises. */
+cn_status cn_rollout_backward(cn_engine* e, const float* const* d_grad_final, uint32_t count);
+/* MSE loss of the last cn_rollout_grad (synchronises). */
+cn_status cn_creature_get_loss(cn_engine* e, uint32_t creature, double* loss);
+/* Gradients of the last backward: dL/dstate0 (C*N*N, n must match; may be NULL) and
+ * dL/dmu, dL/dsigma, dL/dweights (K each; may be NULL).  Host pointers, or device pointers if
+ * device_pointers != 0.  Synchronises. */
+cn_status cn_creature_get_grads(cn_engine* e, uint32_t creature, float* grad_state0, size_t n, float* grad_mu,
+                                float* grad_sigma, float* grad_w, size_t K, int device_pointers);
+
+/* ------------------------------------------------------------------------------------ */
 /* Loader (ml-rs/config.rs)                                                              */
 /* ------------------------------------------------------------------------------------ */
 

@@ -1,0 +1,174 @@
+"""GPU checks of the hand-written adjoint kernels and the checkpointed rollout.
+
+References: torch autograd of the float64 PyTorch restatement of the reference model
+(oracle/torch_ref.py, restating train/diff_lenia_org.py:27-53), float64 central finite
+differences of the same restatement (h = 1e-4 relative to the parameter scale), and a
+dot-product (transpose) test."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import torch_ref  # noqa: E402
+
+DEV = "cuda:0"
+
+
+def _pair(cn, n, C, R, seed, patch):
+    cr = cn.baseline_creature(n, num_channels=C, radius=R, seed=seed, patch=patch)
+    tgt = cn.baseline_creature(n, num_channels=C, radius=R, seed=seed + 100, patch=patch)
+    return cr, tgt
+
+
+def _torch_grads(cr, target_state, steps, temp=1.0):
+    kf, src, dst, mu, sg, w = torch_ref.creature_tensors(cr, DEV)
+    s0 = torch.as_tensor(cr["state"].astype(np.float64), device=DEV).requires_grad_(True)
+    mu, sg, w = (t.clone().requires_grad_(True) for t in (mu, sg, w))
+    xT = torch_ref.rollout(s0, kf, src, dst, mu, sg, w, steps, temp)
+    tgt = torch.as_tensor(target_state.astype(np.float64), device=DEV)
+    loss = ((xT - tgt) ** 2).mean()
+    loss.backward()
+    return float(loss), s0.grad.cpu().numpy(), mu.grad.cpu().numpy(), sg.grad.cpu().numpy(), w.grad.cpu().numpy()
+
+
+def _engine_grads(cn, cr, target_state, steps, ckpt):
+    eng = cn.make_engine(cr)
+    eng.creature_set_target(0, target_state)
+    eng.rollout_grad(steps, ckpt)
+    loss = eng.creature_loss(0)
+    g = eng.creature_grads(0)
+    final = eng.creature_rollout_final(0)
+    eng.close()
+    return loss, g, final
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(np.asarray(a, np.float64) - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("n,C,steps,ckpt", [(64, 3, 1, 1), (64, 3, 6, 2), (128, 2, 8, 3), (64, 4, 5, 5)])
+def test_rollout_grad_matches_torch_fp64(cn, n, C, steps, ckpt):
+    cr, tg = _pair(cn, n, C, R=max(4, n // 6), seed=n + C, patch=n // 2)
+    tgt = tg["state"]
+    loss_t, gs_t, gm_t, gsg_t, gw_t = _torch_grads(cr, tgt, steps)
+    loss, g, _ = _engine_grads(cn, cr, tgt, steps, ckpt)
+    errs = {"loss": abs(loss - loss_t) / loss_t,
+            "state0": _rel(g["state0"].reshape(gs_t.shape), gs_t),
+            "mu": _rel(g["mu"], gm_t), "sigma": _rel(g["sigma"], gsg_t), "weights": _rel(g["weights"], gw_t)}
+    print(f"n={n} C={C} T={steps} ckpt={ckpt}: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+    assert errs["loss"] <= 1e-5
+    for k in ("state0", "mu", "sigma", "weights"):
+        assert errs[k] <= 1e-4, (k, errs[k])
+
+
+def test_checkpoint_stride_does_not_change_gradients(cn):
+    cr, tg = _pair(cn, 64, 3, R=10, seed=5, patch=32)
+    ref = None
+    for ck in (1, 2, 3, 7):
+        _, g, _ = _engine_grads(cn, cr, tg["state"], 7, ck)
+        if ref is None:
+            ref = g
+        else:
+            for k in ("state0", "mu", "sigma", "weights"):
+                assert np.array_equal(g[k], ref[k]), (ck, k)  # recompute is bit-identical
+
+
+def test_rollout_final_equals_run_steps(cn):
+    cr, tg = _pair(cn, 128, 3, R=12, seed=8, patch=64)
+    _, _, final = _engine_grads(cn, cr, tg["state"], 9, 4)
+    eng = cn.make_engine(cr)
+    eng.run_steps(9)
+    assert np.array_equal(final, eng.download_all_channels())
+    eng.close()
+
+
+def test_adjoint_dot_product_vs_fp64_finite_differences(cn):
+    """<J v, w> (J v by float64 central differences of one reference step) = <v, J^T w> (HIP)."""
+    cr, _ = _pair(cn, 64, 3, R=9, seed=21, patch=32)
+    kf, src, dst, mu, sg, w_ = torch_ref.creature_tensors(cr, DEV)
+    s0 = torch.as_tensor(cr["state"].astype(np.float64), device=DEV)
+    rng = np.random.default_rng(0)
+    v = torch.as_tensor(rng.standard_normal(s0.shape), device=DEV)
+    wv = rng.standard_normal(s0.shape).astype(np.float32)
+    h = 1e-4
+    jv = (torch_ref.step(s0 + h * v, kf, src, dst, mu, sg, w_) - torch_ref.step(s0 - h * v, kf, src, dst, mu, sg, w_)) / (2 * h)
+    lhs = float((jv.cpu().numpy() * wv).sum())
+    eng = cn.make_engine(cr)
+    eng.rollout_forward(1, 1)
+    gw = torch.as_tensor(wv, device=DEV).contiguous()
+    eng.rollout_backward([gw])
+    jtw = eng.creature_grads(0)["state0"].reshape(s0.shape)
+    rhs = float((v.cpu().numpy() * jtw).sum())
+    eng.close()
+    print(f"dot-product test: <Jv,w> = {lhs:.9e}, <v,J^T w> = {rhs:.9e}, rel {abs(lhs - rhs) / abs(lhs):.2e}")
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
+def test_param_grads_vs_fp64_finite_differences(cn):
+    cr, tg = _pair(cn, 64, 3, R=9, seed=33, patch=32)
+    steps = 4
+    kf, src, dst, mu, sg, w_ = torch_ref.creature_tensors(cr, DEV)
+    s0 = torch.as_tensor(cr["state"].astype(np.float64), device=DEV)
+    tgt = torch.as_tensor(tg["state"].astype(np.float64), device=DEV)
+
+    def loss_of(m, s, w):
+        return float(((torch_ref.rollout(s0, kf, src, dst, m, s, w, steps) - tgt) ** 2).mean())
+
+    _, g, _ = _engine_grads(cn, cr, tg["state"], steps, 2)
+    for name, idx, scale in (("mu", 0, 0.1), ("sigma", 4, 0.05), ("weights", 8, 0.3)):
+        h = 1e-4 * scale
+        args_p = [mu.clone(), sg.clone(), w_.clone()]
+        args_m = [mu.clone(), sg.clone(), w_.clone()]
+        j = ("mu", "sigma", "weights").index(name)
+        args_p[j][idx] += h
+        args_m[j][idx] -= h
+        fd = (loss_of(*args_p) - loss_of(*args_m)) / (2 * h)
+        print(f"d loss / d {name}[{idx}]: HIP {g[name][idx]:.6e}  FD(fp64) {fd:.6e}")
+        assert abs(g[name][idx] - fd) <= 1e-3 * abs(fd) + 1e-9
+
+
+def test_autograd_function_matches_internal_mse(cn):
+    from cella_nova_amd.autograd import rollout
+    cr, tg = _pair(cn, 64, 3, R=9, seed=41, patch=32)
+    eng = cn.make_engine(cr)
+    s0 = torch.as_tensor(cr["state"], device=DEV).requires_grad_(True)
+    mu, sg, w = (torch.as_tensor(cr[k], device=DEV).requires_grad_(True) for k in ("mu", "sigma", "weights"))
+    xT = rollout(eng, s0, mu, sg, w, steps=5, ckpt_every=2)
+    tgt = torch.as_tensor(tg["state"], device=DEV)
+    loss = ((xT - tgt) ** 2).mean()
+    loss.backward()
+    _, g, final = _engine_grads(cn, cr, tg["state"], 5, 2)
+    assert np.array_equal(xT.detach().cpu().numpy().reshape(-1), final)
+    assert _rel(s0.grad.cpu().numpy().reshape(-1), g["state0"].astype(np.float64)) <= 1e-5
+    for name, t in (("mu", mu), ("sigma", sg), ("weights", w)):
+        assert _rel(t.grad.cpu().numpy(), g[name].astype(np.float64)) <= 1e-5
+    eng.close()
+
+
+def test_differentiable_rollout_1024_c3k9_t64(cn):
+    """Differentiable rollout at 1024^2, C3K9, T=64, checkpoint every 8: MSE to a second seeded
+    creature's state after 64 steps; gradients for the 3x128x128 seed patch and mu/sigma/w
+    against float64 torch autograd of the reference restatement."""
+    n, T = 1024, 64
+    cr, tg = _pair(cn, n, 3, R=13, seed=0, patch=128)
+    eng = cn.make_engine(tg)
+    eng.run_steps(T)
+    target = eng.download_all_channels().reshape(3, n, n)
+    eng.close()
+    loss_t, gs_t, gm_t, gsg_t, gw_t = _torch_grads(cr, target, T)
+    loss, g, _ = _engine_grads(cn, cr, target, T, 8)
+    sl = slice(n // 2 - 64, n // 2 + 64)
+    patch_t = gs_t[:, sl, sl]
+    patch = g["state0"].reshape(3, n, n)[:, sl, sl]
+    errs = {"loss": abs(loss - loss_t) / loss_t, "seed_patch": _rel(patch, patch_t), "mu": _rel(g["mu"], gm_t),
+            "sigma": _rel(g["sigma"], gsg_t), "weights": _rel(g["weights"], gw_t)}
+    print("1024^2 C3K9 T=64 ckpt 8: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+    assert errs["loss"] <= 1e-4
+    for k in ("seed_patch", "mu", "sigma", "weights"):
+        assert errs[k] <= 1e-3, (k, errs[k])
