@@ -101,11 +101,11 @@ k_diffusion_adj(const CreatureDev* __restrict__ descs, int sin, int gin, int N) 
 // Growth adjoint fused with the row R2C transform of gu_k; per-block sums for the
 // parameter gradients.  Grid (N/ROWS, K, creatures).
 // ---------------------------------------------------------------------------------------
-template <int N>
-__global__ void __launch_bounds__(Cfg<N>::ROW_THREADS)
+template <int N, int ER>
+__global__ void __launch_bounds__((RowCfg<N, ER>::THREADS))
 k_growth_adj_row(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) {
-  using CF = Cfg<N>;
-  constexpr int L = CF::L_ROW, E = CF::E_ROW, T = CF::T_ROW, ROWS = CF::ROWS, NT = CF::ROW_THREADS;
+  using CF = RowCfg<N, ER>;
+  constexpr int L = CF::L, E = CF::E, T = CF::T, ROWS = CF::ROWS, NT = CF::THREADS;
   const CreatureDev& d = descs[blockIdx.z];
   if (d.N != N) return;
   const int k = blockIdx.y;
@@ -151,11 +151,10 @@ k_growth_adj_row(const CreatureDev* __restrict__ descs, const float2* __restrict
 // Column adjoint: for one source channel and column tile, sum over the kernels reading that
 // source of FFT_col(rowspec(gu_k)) * conj(K^_k), then one inverse column FFT into spec[c].
 // ---------------------------------------------------------------------------------------
-template <int N, int TC>
-__global__ void __launch_bounds__(TC * Cfg<N>::T_COL) __attribute__((amdgpu_waves_per_eu(4)))
+template <int N, int TC, int EC>
+__global__ void __launch_bounds__((TC * ColCfg<N, EC>::T)) __attribute__((amdgpu_waves_per_eu(4)))
 k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) {
-  using CF = Cfg<N>;
-  constexpr int L = N, E = CF::E_COL, HALF = N / 2;
+  constexpr int L = N, E = ColCfg<N, EC>::E, HALF = N / 2;
   const CreatureDev& d = descs[blockIdx.z];
   if (d.N != N) return;
   const int c = blockIdx.y;
@@ -228,11 +227,11 @@ k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) 
 // ---------------------------------------------------------------------------------------
 // Row C2R of spec[c] (the convolution part of dL/ds) plus the diffusion part -> grad slot.
 // ---------------------------------------------------------------------------------------
-template <int N>
-__global__ void __launch_bounds__(Cfg<N>::ROW_THREADS)
+template <int N, int ER>
+__global__ void __launch_bounds__((RowCfg<N, ER>::THREADS))
 k_row_inv_add(const CreatureDev* __restrict__ descs, int gout, const float2* __restrict__ tw) {
-  using CF = Cfg<N>;
-  constexpr int L = CF::L_ROW, E = CF::E_ROW, T = CF::T_ROW, ROWS = CF::ROWS;
+  using CF = RowCfg<N, ER>;
+  constexpr int L = CF::L, E = CF::E, T = CF::T, ROWS = CF::ROWS;
   const CreatureDev& d = descs[blockIdx.z];
   if (d.N != N) return;
   const int c = blockIdx.y;
@@ -259,10 +258,9 @@ k_row_inv_add(const CreatureDev* __restrict__ descs, int gout, const float2* __r
 // compensation terms.
 // ---------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-k_param_reduce(const CreatureDev* __restrict__ descs, int nblk_of_n_shift) {
+k_param_reduce(const CreatureDev* __restrict__ descs, int nblk) {
   const CreatureDev& d = descs[blockIdx.z];
   const int K = d.K;
-  const int nblk = d.N >> nblk_of_n_shift;  // row blocks per plane
   for (int idx = threadIdx.x; idx < 3 * K; idx += blockDim.x) {
     const int k = idx / 3, j = idx % 3;
     float s = 0.f, comp = 0.f;
@@ -318,21 +316,43 @@ k_mse_grad(const CreatureDev* __restrict__ descs, int sfin, int gout) {
 // ---------------------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------------------
-template <int N, int TC>
+template <int N, int TC, int EC>
 static hipError_t launch_col_adj(const StepLaunch& sl, hipStream_t st) {
   if constexpr (!col_tc_ok<N, TC>()) {
     return hipErrorInvalidValue;
   } else {
-    k_col_adj<N, TC><<<dim3((N / 2) / TC, sl.max_c, sl.count), TC * Cfg<N>::T_COL, 0, st>>>(sl.descs, sl.tw);
+    k_col_adj<N, TC, EC><<<dim3((N / 2) / TC, sl.max_c, sl.count), TC * ColCfg<N, EC>::T, 0, st>>>(sl.descs, sl.tw);
     return hipSuccess;
   }
+}
+
+template <int N, int EC>
+static hipError_t launch_col_adj_tc(const StepLaunch& sl, int tc, hipStream_t st) {
+  switch (tc) {
+    case 2: return launch_col_adj<N, 2, EC>(sl, st);
+    case 4: return launch_col_adj<N, 4, EC>(sl, st);
+    case 8: return launch_col_adj<N, 8, EC>(sl, st);
+    case 16: return launch_col_adj<N, 16, EC>(sl, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int N, int ER>
+static hipError_t launch_adj_rows(const StepLaunch& sl, int gout, hipStream_t st, bool growth) {
+  using CF = RowCfg<N, ER>;
+  if (growth) {
+    k_growth_adj_row<N, ER><<<dim3(CF::BLOCKS, sl.max_k, sl.count), CF::THREADS, 0, st>>>(sl.descs, sl.tw);
+  } else {
+    k_row_inv_add<N, ER><<<dim3(CF::BLOCKS, sl.max_c, sl.count), CF::THREADS, 0, st>>>(sl.descs, gout, sl.tw);
+    k_param_reduce<<<dim3(1, 1, sl.count), 256, 0, st>>>(sl.descs, CF::BLOCKS);
+  }
+  return hipSuccess;
 }
 
 // One adjoint step: recompute the forward intermediates of state slot sl.sin (u, A), then
 // grad slot gin -> gout and parameter-gradient accumulation.
 template <int N>
 static hipError_t launch_adjoint_n(const StepLaunch& sl, int gin, int gout, hipStream_t st) {
-  using CF = Cfg<N>;
   StepLaunch fw = sl;
   fw.write_u = true;
   fw.skip_diffusion = true;
@@ -341,18 +361,23 @@ static hipError_t launch_adjoint_n(const StepLaunch& sl, int gin, int gout, hipS
   if (e != hipSuccess) return e;
   const int tiles_x = (N + kAdjTW - 1) / kAdjTW, tiles_y = (N + kAdjTH - 1) / kAdjTH;
   k_diffusion_adj<<<dim3(tiles_x, tiles_y * sl.max_c, sl.count), kAdjThreads, 0, st>>>(sl.descs, sl.sin, gin, N);
-  k_growth_adj_row<N><<<dim3(CF::ROW_BLOCKS, sl.max_k, sl.count), CF::ROW_THREADS, 0, st>>>(sl.descs, sl.tw);
-  switch (sl.col_tc ? sl.col_tc : CF::TC) {
-    case 2: e = launch_col_adj<N, 2>(sl, st); break;
-    case 4: e = launch_col_adj<N, 4>(sl, st); break;
-    case 8: e = launch_col_adj<N, 8>(sl, st); break;
-    case 16: e = launch_col_adj<N, 16>(sl, st); break;
+  const int re = sl.row_e ? sl.row_e : DefaultGeom<N>::ROW_E;
+  auto rows = [&](bool growth) {
+    switch (re) {
+      case 8: return launch_adj_rows<N, 8>(sl, gout, st, growth);
+      case 16: return launch_adj_rows<N, 16>(sl, gout, st, growth);
+      default: return hipErrorInvalidValue;
+    }
+  };
+  if ((e = rows(true)) != hipSuccess) return e;
+  const int tc = sl.col_tc ? sl.col_tc : DefaultGeom<N>::TC;
+  switch (sl.col_e ? sl.col_e : DefaultGeom<N>::COL_E) {
+    case 8: e = launch_col_adj_tc<N, 8>(sl, tc, st); break;
+    case 16: e = launch_col_adj_tc<N, 16>(sl, tc, st); break;
     default: e = hipErrorInvalidValue;
   }
   if (e != hipSuccess) return e;
-  k_row_inv_add<N><<<dim3(CF::ROW_BLOCKS, sl.max_c, sl.count), CF::ROW_THREADS, 0, st>>>(sl.descs, gout, sl.tw);
-  constexpr int shift = ilog2c(CF::ROWS);
-  k_param_reduce<<<dim3(1, 1, sl.count), 256, 0, st>>>(sl.descs, shift);
+  if ((e = rows(false)) != hipSuccess) return e;
   return hipGetLastError();
 }
 
@@ -370,19 +395,8 @@ hipError_t launch_adjoint(int N, const StepLaunch& sl, int gin, int gout, hipStr
   }
 }
 
-int row_blocks(int N) {
-  switch (N) {
-    case 16: return Cfg<16>::ROW_BLOCKS;
-    case 32: return Cfg<32>::ROW_BLOCKS;
-    case 64: return Cfg<64>::ROW_BLOCKS;
-    case 128: return Cfg<128>::ROW_BLOCKS;
-    case 256: return Cfg<256>::ROW_BLOCKS;
-    case 512: return Cfg<512>::ROW_BLOCKS;
-    case 1024: return Cfg<1024>::ROW_BLOCKS;
-    case 2048: return Cfg<2048>::ROW_BLOCKS;
-    default: return 0;
-  }
-}
+// Upper bound on the row-transform blocks per plane over all geometries (one row per block).
+int row_blocks(int N) { return N; }
 
 hipError_t launch_mse_grad(const CreatureDev* descs, int count, int sfin, int gout, hipStream_t st) {
   k_mse_grad<<<dim3(kLossBlocks, 1, count), kLossThreads, 0, st>>>(descs, sfin, gout);

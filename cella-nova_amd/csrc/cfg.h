@@ -3,19 +3,29 @@
 
 namespace cn {
 
+// Row transforms: an N/2-point complex FFT per row, E values per thread, ROWS rows per block.
+template <int N, int E_REQ>
+struct RowCfg {
+  static constexpr int L = N / 2;
+  static constexpr int E = E_REQ < L ? E_REQ : L;
+  static constexpr int T = L / E;
+  static constexpr int ROWS = T >= 256 ? 1 : ((256 / T) < N ? (256 / T) : N);
+  static constexpr int THREADS = ROWS * T;
+  static constexpr int BLOCKS = N / ROWS;  // blocks per plane
+};
+
+// Column transforms: an N-point complex FFT per column, E values per thread.
+template <int N, int E_REQ>
+struct ColCfg {
+  static constexpr int E = E_REQ < N ? E_REQ : N;
+  static constexpr int T = N / E;
+};
+
+// Built-in launch geometry (used when no tuned plan is given).
 template <int N>
-struct Cfg {
-  // Row transforms: an N/2-point complex FFT per row, E values per thread, ROWS rows per block.
-  static constexpr int L_ROW = N / 2;
-  static constexpr int E_ROW = L_ROW >= 16 ? 16 : L_ROW;
-  static constexpr int T_ROW = L_ROW / E_ROW;
-  static constexpr int ROWS = (256 / T_ROW) < N ? (256 / T_ROW) : N;
-  static constexpr int ROW_THREADS = ROWS * T_ROW;
-  static constexpr int ROW_BLOCKS = N / ROWS;  // blocks per plane
-  // Column transforms: an N-point complex FFT per column, E values per thread.
-  static constexpr int E_COL = N >= 16 ? 16 : N;
-  static constexpr int T_COL = N / E_COL;
-  // default column tile width (complex columns per block of the column passes)
+struct DefaultGeom {
+  static constexpr int ROW_E = 16;
+  static constexpr int COL_E = 16;
   static constexpr int TC_DEF = N >= 2048 ? 4 : (N >= 1024 ? 8 : 16);
   static constexpr int TC = TC_DEF < N / 2 ? TC_DEF : N / 2;
 };

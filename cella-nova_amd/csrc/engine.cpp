@@ -169,6 +169,7 @@ struct SizeClass {
   std::vector<int> members;
   CreatureDev* d_descs = nullptr;
   int max_c = 0, max_k = 0;
+  int row_e = 0, col_e = 0, col_tc = 0;  // launch geometry (0: built-in default)
 };
 
 struct Rollout {
@@ -197,7 +198,6 @@ struct cn_engine {
   std::vector<void*> allocs;
   hipGraphExec_t step_graph[2] = {nullptr, nullptr};
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  int col_tc = 0;           // fused column pass tile width (0: built-in default for N)
   Rollout ro;               // last rollout plan
   bool ro_fwd_done = false; // a rollout forward has run since the last state change
 };
@@ -255,7 +255,9 @@ StepLaunch class_launch(cn_engine* e, const SizeClass& k, int sin, int sout) {
   sl.max_c = k.max_c;
   sl.max_k = k.max_k;
   sl.tw = e->tw;
-  sl.col_tc = (k.N == (int)e->cr[0].N) ? e->col_tc : 0;
+  sl.row_e = k.row_e;
+  sl.col_e = k.col_e;
+  sl.col_tc = k.col_tc;
   return sl;
 }
 
@@ -487,7 +489,6 @@ cn_status create_engine(const cn_engine_desc* descs, uint32_t count, int device,
 
   cn_engine* e = new cn_engine();
   e->device = device;
-  if (const char* tc = std::getenv("CN_COL_TC")) e->col_tc = std::atoi(tc);
   auto fail = [&](cn_status s) {
     release(e);
     return s;
@@ -516,6 +517,10 @@ cn_status create_engine(const cn_engine_desc* descs, uint32_t count, int device,
       e->cls[ci].N = (int)d.n;
     }
     SizeClass& k = e->cls[ci];
+    // Geometry overrides for experiments: CN_ROW_E, CN_COL_E, CN_COL_TC (all size classes).
+    if (const char* v = std::getenv("CN_ROW_E")) k.row_e = std::atoi(v);
+    if (const char* v = std::getenv("CN_COL_E")) k.col_e = std::atoi(v);
+    if (const char* v = std::getenv("CN_COL_TC")) k.col_tc = std::atoi(v);
     c.cls = ci;
     c.idx = (int)k.members.size();
     k.members.push_back((int)i);

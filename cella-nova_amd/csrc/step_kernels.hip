@@ -22,11 +22,11 @@ namespace cn {
 // ---------------------------------------------------------------------------------------
 // Row forward: state rows (real, N) -> half spectrum rows (N/2 complex, slot 0 = (X0, X_{N/2}))
 // ---------------------------------------------------------------------------------------
-template <int N>
-__global__ void __launch_bounds__(Cfg<N>::ROW_THREADS)
+template <int N, int ER>
+__global__ void __launch_bounds__((RowCfg<N, ER>::THREADS))
 k_row_fwd(const CreatureDev* __restrict__ descs, int sin, const float2* __restrict__ tw) {
-  using CF = Cfg<N>;
-  constexpr int L = CF::L_ROW, E = CF::E_ROW, T = CF::T_ROW, ROWS = CF::ROWS;
+  using CF = RowCfg<N, ER>;
+  constexpr int L = CF::L, E = CF::E, T = CF::T, ROWS = CF::ROWS;
   const CreatureDev& d = descs[blockIdx.z];
   if (d.N != N) return;
   const int nrows = d.C * N;
@@ -53,11 +53,10 @@ k_row_fwd(const CreatureDev* __restrict__ descs, int sin, const float2* __restri
 // Fused column pass: forward column FFT of one source tile, then per kernel of that source:
 // product with the kernel's half spectrum and inverse column FFT.
 // ---------------------------------------------------------------------------------------
-template <int N, int TC>
-__global__ void __launch_bounds__(TC * Cfg<N>::T_COL) __attribute__((amdgpu_waves_per_eu(4)))
+template <int N, int TC, int EC>
+__global__ void __launch_bounds__((TC * ColCfg<N, EC>::T)) __attribute__((amdgpu_waves_per_eu(4)))
 k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) {
-  using CF = Cfg<N>;
-  constexpr int L = N, E = CF::E_COL, HALF = N / 2;
+  constexpr int L = N, E = ColCfg<N, EC>::E, HALF = N / 2;
   const CreatureDev& d = descs[blockIdx.z];
   if (d.N != N) return;
   const int c = blockIdx.y;
@@ -139,11 +138,11 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
 // ---------------------------------------------------------------------------------------
 // Row inverse (C2R) of every kernel's result + growth + weighted sum + affinity exp.
 // ---------------------------------------------------------------------------------------
-template <int N, bool WRITE_U>
-__global__ void __launch_bounds__(Cfg<N>::ROW_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
+template <int N, int ER, bool WRITE_U>
+__global__ void __launch_bounds__((RowCfg<N, ER>::THREADS)) __attribute__((amdgpu_waves_per_eu(4)))
 k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) {
-  using CF = Cfg<N>;
-  constexpr int L = CF::L_ROW, E = CF::E_ROW, T = CF::T_ROW, ROWS = CF::ROWS;
+  using CF = RowCfg<N, ER>;
+  constexpr int L = CF::L, E = CF::E, T = CF::T, ROWS = CF::ROWS;
   const CreatureDev& d = descs[blockIdx.z];
   if (d.N != N) return;
   const int c = blockIdx.y;  // destination channel
@@ -313,39 +312,68 @@ k_mass(const CreatureDev* __restrict__ descs, int slot, double2* __restrict__ pa
 // ---------------------------------------------------------------------------------------
 static hipError_t record(hipEvent_t ev, hipStream_t st) { return ev ? hipEventRecord(ev, st) : hipSuccess; }
 
-template <int N, int TC>
+template <int N, int TC, int EC>
 static hipError_t launch_col(const StepLaunch& sl, hipStream_t st) {
   if constexpr (!col_tc_ok<N, TC>()) {
     return hipErrorInvalidValue;
   } else {
     const dim3 g_col((N / 2) / TC, sl.max_c, sl.count);
-    k_col_fused<N, TC><<<g_col, TC * Cfg<N>::T_COL, 0, st>>>(sl.descs, sl.tw);
+    k_col_fused<N, TC, EC><<<g_col, TC * ColCfg<N, EC>::T, 0, st>>>(sl.descs, sl.tw);
     return hipSuccess;
+  }
+}
+
+template <int N, int EC>
+static hipError_t launch_col_tc(const StepLaunch& sl, int tc, hipStream_t st) {
+  switch (tc) {
+    case 2: return launch_col<N, 2, EC>(sl, st);
+    case 4: return launch_col<N, 4, EC>(sl, st);
+    case 8: return launch_col<N, 8, EC>(sl, st);
+    case 16: return launch_col<N, 16, EC>(sl, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int N, int ER>
+static hipError_t launch_rows(const StepLaunch& sl, hipStream_t st, bool fwd) {
+  using CF = RowCfg<N, ER>;
+  if (fwd) {
+    const dim3 g_row((sl.max_c * N + CF::ROWS - 1) / CF::ROWS, 1, sl.count);
+    k_row_fwd<N, ER><<<g_row, CF::THREADS, 0, st>>>(sl.descs, sl.sin, sl.tw);
+  } else {
+    const dim3 g_inv(CF::BLOCKS, sl.max_c, sl.count);
+    if (sl.write_u)
+      k_row_inv_growth<N, ER, true><<<g_inv, CF::THREADS, 0, st>>>(sl.descs, sl.tw);
+    else
+      k_row_inv_growth<N, ER, false><<<g_inv, CF::THREADS, 0, st>>>(sl.descs, sl.tw);
+  }
+  return hipSuccess;
+}
+
+template <int N>
+static hipError_t launch_rows_e(const StepLaunch& sl, hipStream_t st, bool fwd) {
+  switch (sl.row_e ? sl.row_e : DefaultGeom<N>::ROW_E) {
+    case 8: return launch_rows<N, 8>(sl, st, fwd);
+    case 16: return launch_rows<N, 16>(sl, st, fwd);
+    default: return hipErrorInvalidValue;
   }
 }
 
 template <int N>
 static hipError_t launch_step_n(const StepLaunch& sl, hipStream_t st) {
-  using CF = Cfg<N>;
   hipError_t e;
   if ((e = record(sl.ev[0], st)) != hipSuccess) return e;
-  const dim3 g_row((sl.max_c * N + CF::ROWS - 1) / CF::ROWS, 1, sl.count);
-  k_row_fwd<N><<<g_row, CF::ROW_THREADS, 0, st>>>(sl.descs, sl.sin, sl.tw);
+  if ((e = launch_rows_e<N>(sl, st, true)) != hipSuccess) return e;
   if ((e = record(sl.ev[1], st)) != hipSuccess) return e;
-  switch (sl.col_tc ? sl.col_tc : CF::TC) {
-    case 2: e = launch_col<N, 2>(sl, st); break;
-    case 4: e = launch_col<N, 4>(sl, st); break;
-    case 8: e = launch_col<N, 8>(sl, st); break;
-    case 16: e = launch_col<N, 16>(sl, st); break;
+  const int tc = sl.col_tc ? sl.col_tc : DefaultGeom<N>::TC;
+  switch (sl.col_e ? sl.col_e : DefaultGeom<N>::COL_E) {
+    case 8: e = launch_col_tc<N, 8>(sl, tc, st); break;
+    case 16: e = launch_col_tc<N, 16>(sl, tc, st); break;
     default: e = hipErrorInvalidValue;
   }
   if (e != hipSuccess) return e;
   if ((e = record(sl.ev[2], st)) != hipSuccess) return e;
-  const dim3 g_inv(N / CF::ROWS, sl.max_c, sl.count);
-  if (sl.write_u)
-    k_row_inv_growth<N, true><<<g_inv, CF::ROW_THREADS, 0, st>>>(sl.descs, sl.tw);
-  else
-    k_row_inv_growth<N, false><<<g_inv, CF::ROW_THREADS, 0, st>>>(sl.descs, sl.tw);
+  if ((e = launch_rows_e<N>(sl, st, false)) != hipSuccess) return e;
   if ((e = record(sl.ev[3], st)) != hipSuccess) return e;
   if (!sl.skip_diffusion) {
     const int tiles_x = (N + kDiffTW - 1) / kDiffTW, tiles_y = (N + kDiffTH - 1) / kDiffTH;
