@@ -165,8 +165,7 @@ k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) 
   int t = threadIdx.x / TC;
   const int x = x0 + lane;
   const bool edge = (x0 == 0);
-  __shared__ float2 lds[L * TC];
-  __shared__ float2 ldsB[N];
+  __shared__ float2 lds[LdsMap<L, TC, true>::size];
 
   float2 acc[E];
 #pragma unroll
@@ -186,7 +185,7 @@ k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) 
       __syncthreads();
       if (lane == 0) {
 #pragma unroll
-        for (int i = 0; i < E; ++i) ldsB[out_pos<L, E, false>(t, i)] = v[i];
+        for (int i = 0; i < E; ++i) lds[LdsMap<L, TC, true>::at(out_pos<L, E, false>(t, i), 0)] = v[i];
       }
       __syncthreads();
       if (lane == 0) {
@@ -195,7 +194,7 @@ k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) 
         for (int i = 0; i < E; ++i) {
           const int y = out_pos<L, E, false>(t, i);
           const float2 F = v[i];
-          const float2 G = ldsB[(N - y) & (N - 1)];
+          const float2 G = lds[LdsMap<L, TC, true>::at((N - y) & (N - 1), 0)];
           const float2 a = make_float2(0.5f * (F.x + G.x), 0.5f * (F.y - G.y));
           const float2 dd = make_float2(F.x - G.x, F.y + G.y);
           const float2 b = make_float2(0.5f * dd.y, -0.5f * dd.x);
@@ -318,7 +317,7 @@ k_mse_grad(const CreatureDev* __restrict__ descs, int sfin, int gout) {
 // ---------------------------------------------------------------------------------------
 template <int N, int TC, int EC>
 static hipError_t launch_col_adj(const StepLaunch& sl, hipStream_t st) {
-  if constexpr (!col_tc_ok<N, TC>()) {
+  if constexpr (!col_tc_ok<N, TC, EC>()) {
     return hipErrorInvalidValue;
   } else {
     k_col_adj<N, TC, EC><<<dim3((N / 2) / TC, sl.max_c, sl.count), TC * ColCfg<N, EC>::T, 0, st>>>(sl.descs, sl.tw);

@@ -30,10 +30,11 @@ struct DefaultGeom {
   static constexpr int TC = TC_DEF < N / 2 ? TC_DEF : N / 2;
 };
 
-// Column tile widths a launcher may pick for size N (LDS: N*TC*8 + N*8 bytes <= 160 KiB).
-template <int N, int TC>
+// Column geometries a launcher may pick for size N: at most 1024 threads per block and the
+// padded exchange buffer (N + N/16) * TC * 8 bytes within 160 KiB of LDS.
+template <int N, int TC, int EC>
 constexpr bool col_tc_ok() {
-  return TC <= N / 2 && (long)N * TC * 8 + (long)N * 8 <= 160 * 1024;
+  return TC <= N / 2 && TC * ColCfg<N, EC>::T <= 1024 && (long)(N + N / 16) * TC * 8 <= 160 * 1024;
 }
 
 // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs (separate L2s); give

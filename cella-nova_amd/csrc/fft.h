@@ -155,14 +155,17 @@ struct Plan {
 };
 
 // LDS addressing of element `pos` of transform `lane` (0 <= lane < NT) in a block that holds
-// NT independent transforms.  INTERLEAVED: pos*NT + lane (lane-consecutive threads touch
-// consecutive addresses; used for column tiles).  Otherwise lane*(L+PAD) + pos.
+// NT independent transforms.  INTERLEAVED: (pos + pad)*NT + lane (lane-consecutive threads
+// touch consecutive addresses; used for column tiles).  Otherwise lane*LP + pos + pad.
+// pad = pos/16 inserts one element every 16: the first pass of a transform writes its outputs
+// at stride R (16), which without padding puts every lane of a wave on the same LDS banks.
 template <int L, int NT, bool INTERLEAVED>
 struct LdsMap {
-  static constexpr int PAD = INTERLEAVED ? 0 : 0;
-  static constexpr int size = INTERLEAVED ? L * NT : NT * (L + PAD);
+  static constexpr int LP = L + L / 16;
+  static constexpr int size = NT * LP;
   static __device__ __forceinline__ int at(int pos, int lane) {
-    return INTERLEAVED ? pos * NT + lane : lane * (L + PAD) + pos;
+    const int pp = pos + (pos >> 4);
+    return INTERLEAVED ? pp * NT + lane : lane * LP + pp;
   }
 };
 

@@ -73,59 +73,45 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
 #pragma unroll
   for (int i = 0; i < E; ++i) v[i] = sp[(size_t)in_pos<L, E, false>(t, i) * HALF + x];
 
-  __shared__ float2 lds[L * TC];
-  __shared__ float2 ldsB[N];
+  __shared__ float2 lds[LdsMap<L, TC, true>::size];
   fft_run<L, E, TC, true, false, kTwN>(v, lds, t, lane, tw);
 
-  // Tile 0, column 0 carries two real columns packed as (DC column) + i (Nyquist column):
-  // split F into A = FFT(DC column) and B = FFT(Nyquist column) once per source.
+  // Tile 0, column 0 carries two real columns packed as (DC column) + i (Nyquist column).
+  // Per kernel, lane 0 splits its spectrum F into A = FFT(DC column) and B = FFT(Nyquist
+  // column) through the exchange buffer (free between transforms), multiplies each by its own
+  // kernel column and re-packs A K_dc + i B K_nyq.
   const bool edge = (x0 == 0);
-  if (edge) {
-    if (lane == 0) {
-#pragma unroll
-      for (int i = 0; i < E; ++i) ldsB[out_pos<L, E, false>(t, i)] = v[i];
-    }
-    __syncthreads();
-    float2 bb[E];
-    if (lane == 0) {
-#pragma unroll
-      for (int i = 0; i < E; ++i) {
-        const int y = out_pos<L, E, false>(t, i);
-        const float2 F = v[i];
-        const float2 G = ldsB[(N - y) & (N - 1)];
-        const float2 a = make_float2(0.5f * (F.x + G.x), 0.5f * (F.y - G.y));   // (F + conj G)/2
-        const float2 dd = make_float2(F.x - G.x, F.y + G.y);                     // F - conj G
-        bb[i] = make_float2(0.5f * dd.y, -0.5f * dd.x);                          // dd / (2i)
-        v[i] = a;
-      }
-    }
-    __syncthreads();
-    if (lane == 0) {
-#pragma unroll
-      for (int i = 0; i < E; ++i) ldsB[out_pos<L, E, false>(t, i)] = bb[i];
-    }
-    __syncthreads();
-  }
-
   for (int kk = kb; kk < ke; ++kk) {
     const int k = d.src_k[kk];
     t = launder(t);
     const float2* kh = reinterpret_cast<const float2*>(d.khalf) + (size_t)k * N * HALF;
     float2 w[E];
-    if (edge && lane == 0) {
-      const float2* kn = reinterpret_cast<const float2*>(d.knyq) + (size_t)k * N;
 #pragma unroll
-      for (int i = 0; i < E; ++i) {
-        const int y = out_pos<L, E, false>(t, i);
-        const float2 pa = c_mul(v[i], kh[(size_t)y * HALF]);
-        const float2 pb = c_mul(ldsB[y], kn[y]);
-        w[i] = make_float2(pa.x - pb.y, pa.y + pb.x);  // pa + i*pb
+    for (int i = 0; i < E; ++i) {
+      const int y = out_pos<L, E, false>(t, i);
+      w[i] = c_mul(v[i], kh[(size_t)y * HALF + x]);
+    }
+    if (edge) {
+      __syncthreads();
+      if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) lds[LdsMap<L, TC, true>::at(out_pos<L, E, false>(t, i), 0)] = v[i];
       }
-    } else {
+      __syncthreads();
+      if (lane == 0) {
+        const float2* kn = reinterpret_cast<const float2*>(d.knyq) + (size_t)k * N;
 #pragma unroll
-      for (int i = 0; i < E; ++i) {
-        const int y = out_pos<L, E, false>(t, i);
-        w[i] = c_mul(v[i], kh[(size_t)y * HALF + x]);
+        for (int i = 0; i < E; ++i) {
+          const int y = out_pos<L, E, false>(t, i);
+          const float2 F = v[i];
+          const float2 G = lds[LdsMap<L, TC, true>::at((N - y) & (N - 1), 0)];
+          const float2 a = make_float2(0.5f * (F.x + G.x), 0.5f * (F.y - G.y));  // (F + conj G)/2
+          const float2 dd = make_float2(F.x - G.x, F.y + G.y);                    // F - conj G
+          const float2 b = make_float2(0.5f * dd.y, -0.5f * dd.x);                // dd / (2i)
+          const float2 pa = c_mul(a, kh[(size_t)y * HALF]);
+          const float2 pb = c_mul(b, kn[y]);
+          w[i] = make_float2(pa.x - pb.y, pa.y + pb.x);  // pa + i*pb
+        }
       }
     }
     fft_run<L, E, TC, true, true, kTwN>(w, lds, t, lane, tw);
@@ -314,7 +300,7 @@ static hipError_t record(hipEvent_t ev, hipStream_t st) { return ev ? hipEventRe
 
 template <int N, int TC, int EC>
 static hipError_t launch_col(const StepLaunch& sl, hipStream_t st) {
-  if constexpr (!col_tc_ok<N, TC>()) {
+  if constexpr (!col_tc_ok<N, TC, EC>()) {
     return hipErrorInvalidValue;
   } else {
     const dim3 g_col((N / 2) / TC, sl.max_c, sl.count);
