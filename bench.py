@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--radius", type=int, default=26)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run a device-to-device copy of this many MiB (PMC calibration only)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU only; multi-GPU runs are not part of this build "
@@ -76,6 +78,13 @@ def main():
         sys.exit(2)
 
     pkg = ge.load_package()
+    if args.copy_calibration_mb:
+        import torch
+        a = torch.ones(args.copy_calibration_mb << 18, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
     n, C, R = args.n, 3, args.radius
     cr = pkg.baseline_creature(n, num_channels=C, radius=R, seed=0)
     K = cr["num_kernels"]

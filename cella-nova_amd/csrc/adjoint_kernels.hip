@@ -123,7 +123,7 @@ k_growth_adj_row(const CreatureDev* __restrict__ descs, const float2* __restrict
     const int n = in_pos<L, E, false>(t, i);
     const float2 uu = u[n], gg = ga[n];
     const float d0 = uu.x - mu, d1 = uu.y - mu;
-    const float e0 = 2.0f * expf(-0.5f * d0 * d0 * is2), e1 = 2.0f * expf(-0.5f * d1 * d1 * is2);  // G + 1
+    const float e0 = 2.0f * exp_fast(-0.5f * d0 * d0 * is2), e1 = 2.0f * exp_fast(-0.5f * d1 * d1 * is2);  // G + 1
     const float gu0 = wk * gg.x * e0 * (-d0 * is2), gu1 = wk * gg.y * e1 * (-d1 * is2);
     v[i] = make_float2(gu0, gu1);
     smu -= gu0 + gu1;

@@ -41,6 +41,20 @@ __device__ __forceinline__ float2 c_scale(float2 a, float s) { return make_float
 
 constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
 
+// exp(x) from the hardware base-2 exponential (v_exp_f32, ~1 ulp) with the argument split
+// x*log2(e) = p + e carried in extended precision: p = RN(x*L2E), e = the rounding error of
+// that product (FMA) plus x*(log2(e) - L2E); exp(x) = 2^p * 2^e = 2^p * (1 + e ln 2 + O(e^2)),
+// |e| <= 2^-24 |p| + 2^-50 |x|, so the linear term is enough.  Six VALU operations, about
+// 2 ulp; inputs whose result under/overflows single precision give 0 / inf like expf.
+__device__ __forceinline__ float exp_fast(float x) {
+  constexpr float L2E = 1.44269502163f;        // RN(log2 e)
+  constexpr float L2E_LO = 1.9259629891e-08f;     // log2(e) - L2E
+  const float p = x * L2E;
+  const float e = fmaf(x, L2E, -p) + x * L2E_LO;
+  const float r = __builtin_amdgcn_exp2f(p);
+  return fmaf(r, e * 0.693147180560f, r);
+}
+
 // cos(2*pi*k/16), sin(2*pi*k/16) for k = 0..3 (enough for every radix-16 sub-twiddle
 // after the special angles 0, 1/8, 1/4, 3/8 are handled exactly).
 constexpr float kC16_1 = 0.92387953251128675613f;  // cos(pi/8)

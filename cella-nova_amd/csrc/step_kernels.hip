@@ -158,8 +158,8 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
           reinterpret_cast<float2*>(d.u + ((size_t)k * N + y) * N)[n] = v[i];
         }
         const float d0 = u0 - mu, d1 = u1 - mu;
-        const float g0 = 2.0f * expf(-0.5f * d0 * d0 * is2) - 1.0f;
-        const float g1 = 2.0f * expf(-0.5f * d1 * d1 * is2) - 1.0f;
+        const float g0 = 2.0f * exp_fast(-0.5f * d0 * d0 * is2) - 1.0f;
+        const float g1 = 2.0f * exp_fast(-0.5f * d1 * d1 * is2) - 1.0f;
         acc[i].x += g0 * wk;
         acc[i].y += g1 * wk;
       }
@@ -169,7 +169,7 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const int n = out_pos<L, E, true>(t, i);
-      Ao[n] = make_float2(expf(temp * acc[i].x), expf(temp * acc[i].y));
+      Ao[n] = make_float2(exp_fast(temp * acc[i].x), exp_fast(temp * acc[i].y));
     }
   }
 }
@@ -180,12 +180,12 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
 // Mass conservation in FP32.  Exactly, sum_p new_p = sum_n (s_n / Z_n) * sum_{p in N(n)} A_p
 // = sum_n s_n.  Plain FP32 breaks the identity by a systematic ~5e-8 per step (the reference's
 // f32 CPU step drifts the same way): Z_n is not exactly the sum of the A_p that later receive
-// q_n, q_n*Z_n != s_n, and in smooth regions these roundings repeat with the same sign.  Here
-// Z and the gather sums use error-free TwoSum chains (hi + lo), each site keeps the exact
-// residual r_n = s_n - q_n*(Z_hi + Z_lo) of its division (FMA), and adds it back to its own
-// output.  What is left is the final rounding of each output (random sign).
+// q_n, and in smooth regions these roundings repeat with the same sign.  Here Z is an
+// error-free (hi + lo) sum (TwoSum, horizontal 3-sums then vertical), each site keeps the exact
+// residual r_n = s_n - q_n*(Z_hi + Z_lo) of its division (FMA) and adds it back to its own
+// output.  What is left are the roundings of the gather sums and outputs (no systematic sign).
 // ---------------------------------------------------------------------------------------
-constexpr int kDiffTW = 64, kDiffTH = 16, kDiffThreads = 256;
+constexpr int kDiffTW = 64, kDiffThreads = 256;
 
 __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
   s = a + b;
@@ -193,63 +193,81 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
   e = (a - (s - bb)) + (b - bb);
 }
 
+// Maps a linear block index onto (tile_x, tile_y) so that each XCD (blocks are dealt to the 8
+// XCDs round-robin) works on a contiguous row-major run of tiles: neighbouring tiles share the
+// cache lines of their halo columns, which then come from the same L2.
+__device__ __forceinline__ void xcd_tile2d(int gx, int gy, int& tx, int& ty) {
+  const int lin = blockIdx.x + gx * blockIdx.y, total = gx * gy;
+  int m = lin;
+  if (total % 8 == 0) m = (lin % 8) * (total / 8) + lin / 8;
+  tx = m % gx;
+  ty = m / gx;
+}
+
+template <int TH>
 __global__ void __launch_bounds__(kDiffThreads)
 k_diffusion(const CreatureDev* __restrict__ descs, int sin, int sout, int N) {
-  constexpr int TW = kDiffTW, TH = kDiffTH;
+  constexpr int TW = kDiffTW;
   const CreatureDev& d = descs[blockIdx.z];
   if (d.N != N) return;
   const int tiles_y = (N + TH - 1) / TH;
-  const int c = blockIdx.y / tiles_y;
+  int bx, by;
+  xcd_tile2d(gridDim.x, gridDim.y, bx, by);
+  const int c = by / tiles_y;
   if (c >= d.C) return;
-  const int y0 = (blockIdx.y % tiles_y) * TH;
-  const int x0 = blockIdx.x * TW;
+  const int y0 = (by % tiles_y) * TH;
+  const int x0 = bx * TW;
   const int mask = N - 1;
   const size_t plane = (size_t)N * N;
   const float* A = d.A + c * plane;
   const float* s = slot_ptr(d, sin) + c * plane;
   float* o = slot_ptr(d, sout) + c * plane;
 
-  __shared__ float sA[TH + 4][TW + 5];
-  __shared__ float sQ[TH + 2][TW + 3];
-  __shared__ float sR[TH + 2][TW + 3];
-  const int tid = threadIdx.x;
-  for (int i = tid; i < (TH + 4) * (TW + 4); i += kDiffThreads) {
-    const int r = i / (TW + 4), cc = i % (TW + 4);
-    const int gy = (y0 - 2 + r) & mask, gx = (x0 - 2 + cc) & mask;
-    sA[r][cc] = A[(size_t)gy * N + gx];
+  __shared__ float sA[TH + 4][TW + 5];   // A, rows y0-2.., cols x0-2..
+  __shared__ float sHh[TH + 4][TW + 3];  // horizontal 3-sums of A (hi), cols x0-1..
+  __shared__ float sHl[TH + 4][TW + 3];  //   and their low parts
+  __shared__ float sQ[TH + 2][TW + 3];   // q = s / Z, rows y0-1.., cols x0-1..
+  __shared__ float sR[TH][TW + 1];       // division residuals of the interior
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4 threads
+
+  for (int r = ty; r < TH + 4; r += 4) {
+    const size_t row = (size_t)((y0 - 2 + r) & mask) * N;
+    sA[r][tx] = A[row + ((x0 - 2 + tx) & mask)];
+    if (tx < 4) sA[r][tx + 64] = A[row + ((x0 + 62 + tx) & mask)];
   }
   __syncthreads();
-  for (int i = tid; i < (TH + 2) * (TW + 2); i += kDiffThreads) {
-    const int r = i / (TW + 2), cc = i % (TW + 2);
-    float zh = sA[r][cc], zl = 0.f;
-#pragma unroll
-    for (int j = 1; j < 9; ++j) {
-      float e;
-      two_sum(zh, sA[r + j / 3][cc + j % 3], zh, e);
-      zl += e;
+  for (int r = ty; r < TH + 4; r += 4) {
+    for (int cc = tx; cc < TW + 2; cc += 64) {
+      float h, e1, e2;
+      two_sum(sA[r][cc], sA[r][cc + 1], h, e1);
+      two_sum(h, sA[r][cc + 2], h, e2);
+      sHh[r][cc] = h;
+      sHl[r][cc] = e1 + e2;
     }
-    const int gy = (y0 - 1 + r) & mask, gx = (x0 - 1 + cc) & mask;
-    const float sv = s[(size_t)gy * N + gx];
-    const float q = sv / zh;
-    sQ[r][cc] = q;
-    sR[r][cc] = fmaf(-q, zl, fmaf(-q, zh, sv));  // s - q*(zh + zl)
   }
   __syncthreads();
-  for (int i = tid; i < TH * TW; i += kDiffThreads) {
-    const int r = i / TW, cc = i % TW;
-    const int gy = y0 + r, gx = x0 + cc;
-    float sh = sQ[r][cc], sl = 0.f;
-#pragma unroll
-    for (int j = 1; j < 9; ++j) {
-      float e;
-      two_sum(sh, sQ[r + j / 3][cc + j % 3], sh, e);
-      sl += e;
+  for (int r = ty; r < TH + 2; r += 4) {
+    const size_t row = (size_t)((y0 - 1 + r) & mask) * N;
+    for (int cc = tx; cc < TW + 2; cc += 64) {
+      float zh, e1, e2;
+      two_sum(sHh[r][cc], sHh[r + 1][cc], zh, e1);
+      two_sum(zh, sHh[r + 2][cc], zh, e2);
+      const float zl = (e1 + e2) + (sHl[r][cc] + sHl[r + 1][cc] + sHl[r + 2][cc]);
+      const float sv = s[row + ((x0 - 1 + cc) & mask)];
+      const float q = sv / zh;
+      sQ[r][cc] = q;
+      if (r >= 1 && r <= TH && cc >= 1 && cc <= TW) sR[r - 1][cc - 1] = fmaf(-q, zl, fmaf(-q, zh, sv));
     }
-    const float a = sA[r + 2][cc + 2];
-    const float t = a * sh;
-    const float err = fmaf(a, sh, -t);
-    const float v = t + (err + fmaf(a, sl, sR[r + 1][cc + 1]));
-    if (gy < N && gx < N) o[(size_t)gy * N + gx] = v;
+  }
+  __syncthreads();
+  for (int r = ty; r < TH; r += 4) {
+    const int gy = y0 + r, gx = x0 + tx;
+    float acc = 0.f;
+#pragma unroll
+    for (int dr = 0; dr < 3; ++dr)
+#pragma unroll
+      for (int dc = 0; dc < 3; ++dc) acc += sQ[r + dr][tx + dc];
+    if (gy < N && gx < N) o[(size_t)gy * N + gx] = fmaf(sA[r + 2][tx + 2], acc, sR[r][tx]);
   }
 }
 
@@ -362,9 +380,15 @@ static hipError_t launch_step_n(const StepLaunch& sl, hipStream_t st) {
   if ((e = launch_rows_e<N>(sl, st, false)) != hipSuccess) return e;
   if ((e = record(sl.ev[3], st)) != hipSuccess) return e;
   if (!sl.skip_diffusion) {
-    const int tiles_x = (N + kDiffTW - 1) / kDiffTW, tiles_y = (N + kDiffTH - 1) / kDiffTH;
+    const int th = sl.diff_th ? sl.diff_th : 16;
+    const int tiles_x = (N + kDiffTW - 1) / kDiffTW, tiles_y = (N + th - 1) / th;
     const dim3 g_dif(tiles_x, tiles_y * sl.max_c, sl.count);
-    k_diffusion<<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N);
+    if (th == 32)
+      k_diffusion<32><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N);
+    else if (th == 16)
+      k_diffusion<16><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N);
+    else
+      return hipErrorInvalidValue;
   }
   if ((e = record(sl.ev[4], st)) != hipSuccess) return e;
   return hipGetLastError();
