@@ -169,7 +169,7 @@ struct SizeClass {
   std::vector<int> members;
   CreatureDev* d_descs = nullptr;
   int max_c = 0, max_k = 0;
-  int row_e = 0, col_e = 0, col_tc = 0, diff_th = 0;  // launch geometry (0: built-in default)
+  int row_e = 0, col_e = 0, col_tc = 0;  // launch geometry (0: built-in default)
 };
 
 struct Rollout {
@@ -258,7 +258,6 @@ StepLaunch class_launch(cn_engine* e, const SizeClass& k, int sin, int sout) {
   sl.row_e = k.row_e;
   sl.col_e = k.col_e;
   sl.col_tc = k.col_tc;
-  sl.diff_th = k.diff_th;
   return sl;
 }
 
@@ -522,7 +521,6 @@ cn_status create_engine(const cn_engine_desc* descs, uint32_t count, int device,
     if (const char* v = std::getenv("CN_ROW_E")) k.row_e = std::atoi(v);
     if (const char* v = std::getenv("CN_COL_E")) k.col_e = std::atoi(v);
     if (const char* v = std::getenv("CN_COL_TC")) k.col_tc = std::atoi(v);
-    if (const char* v = std::getenv("CN_DIFF_TH")) k.diff_th = std::atoi(v);
     c.cls = ci;
     c.idx = (int)k.members.size();
     k.members.push_back((int)i);

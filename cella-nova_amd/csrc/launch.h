@@ -17,7 +17,6 @@ struct StepLaunch {
   // Launch geometry (0: built-in default): values per thread of the row and column FFTs and
   // the column tile width of the column passes.
   int row_e = 0, col_e = 0, col_tc = 0;
-  int diff_th = 0;  // diffusion tile height (0: default)
   // Optional per-kernel timing events: ev[0] before the row pass, ev[1] after it, ev[2] after
   // the column pass, ev[3] after the inverse-row + growth pass, ev[4] after the diffusion.
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};

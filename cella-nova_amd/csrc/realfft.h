@@ -37,27 +37,46 @@ __device__ __forceinline__ void r2c_finish(const float2* v, float2* lds, int t, 
   }
 }
 
-// Loads a half-spectrum row X (L = N/2 complex) and forms the L-point input Z2 of the inverse
-// transform whose unnormalised output is z[n] = x[2n] + i x[2n+1] with x the unnormalised
-// N-point real inverse of X.
+// Loads a half-spectrum row X (L = N/2 complex) into the pairs (X[p], X[L-p]) that the inverse
+// transform's first pass needs; c2r_form turns them into the L-point input Z2 of the inverse
+// transform whose unnormalised output is z[n] = x[2n] + i x[2n+1], x being the unnormalised
+// N-point real inverse of X.  Split in two so that callers can issue the loads of the next
+// row early (software prefetch).
 template <int N, int E>
-__device__ __forceinline__ void c2r_start(const float2* __restrict__ X, int t, const float2* __restrict__ tw,
-                                          float2* v) {
+__device__ __forceinline__ void c2r_load(const float2* __restrict__ X, int t, float2* xp, float2* xq) {
   constexpr int L = N / 2;
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const int p = in_pos<L, E, true>(t, i);
-    const float2 xp = X[p];
+    xp[i] = X[p];
+    xq[i] = X[(L - p) & (L - 1)];
+  }
+}
+
+template <int N, int E>
+__device__ __forceinline__ void c2r_form(const float2* xp, const float2* xq, int t, const float2* __restrict__ tw,
+                                         float2* v) {
+  constexpr int L = N / 2;
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int p = in_pos<L, E, true>(t, i);
     if (p == 0) {
-      v[i] = make_float2(xp.x + xp.y, xp.x - xp.y);
+      v[i] = make_float2(xp[i].x + xp[i].y, xp[i].x - xp[i].y);
     } else {
-      const float2 xq = X[L - p];
-      const float2 e2 = make_float2(xp.x + xq.x, xp.y - xq.y);  // xp + conj(xq)
-      const float2 dd = make_float2(xp.x - xq.x, xp.y + xq.y);  // xp - conj(xq)
-      const float2 o2 = c_mulc(dd, tw[p * (kTwN / N)]);          // * W_N^{-p}
-      v[i] = make_float2(e2.x - o2.y, e2.y + o2.x);             // e2 + i*o2
+      const float2 e2 = make_float2(xp[i].x + xq[i].x, xp[i].y - xq[i].y);  // xp + conj(xq)
+      const float2 dd = make_float2(xp[i].x - xq[i].x, xp[i].y + xq[i].y);  // xp - conj(xq)
+      const float2 o2 = c_mulc(dd, tw[p * (kTwN / N)]);                      // * W_N^{-p}
+      v[i] = make_float2(e2.x - o2.y, e2.y + o2.x);                         // e2 + i*o2
     }
   }
+}
+
+template <int N, int E>
+__device__ __forceinline__ void c2r_start(const float2* __restrict__ X, int t, const float2* __restrict__ tw,
+                                          float2* v) {
+  float2 xp[E], xq[E];
+  c2r_load<N, E>(X, t, xp, xq);
+  c2r_form<N, E>(xp, xq, t, tw, v);
 }
 
 }  // namespace cn

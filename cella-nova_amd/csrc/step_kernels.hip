@@ -73,6 +73,15 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
 #pragma unroll
   for (int i = 0; i < E; ++i) v[i] = sp[(size_t)in_pos<L, E, false>(t, i) * HALF + x];
 
+  // Kernel spectra are prefetched one kernel ahead: the loads of kernel k+1's tile are in
+  // flight while kernel k's inverse transform runs (the first ones during the forward one).
+  const float2* khalf = reinterpret_cast<const float2*>(d.khalf);
+  float2 nk[E];
+  {
+    const float2* kh0 = khalf + (size_t)d.src_k[kb] * N * HALF;
+#pragma unroll
+    for (int i = 0; i < E; ++i) nk[i] = kh0[(size_t)out_pos<L, E, false>(t, i) * HALF + x];
+  }
   __shared__ float2 lds[LdsMap<L, TC, true>::size];
   fft_run<L, E, TC, true, false, kTwN>(v, lds, t, lane, tw);
 
@@ -84,12 +93,14 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
   for (int kk = kb; kk < ke; ++kk) {
     const int k = d.src_k[kk];
     t = launder(t);
-    const float2* kh = reinterpret_cast<const float2*>(d.khalf) + (size_t)k * N * HALF;
+    const float2* kh = khalf + (size_t)k * N * HALF;
     float2 w[E];
 #pragma unroll
-    for (int i = 0; i < E; ++i) {
-      const int y = out_pos<L, E, false>(t, i);
-      w[i] = c_mul(v[i], kh[(size_t)y * HALF + x]);
+    for (int i = 0; i < E; ++i) w[i] = c_mul(v[i], nk[i]);
+    if (kk + 1 < ke) {
+      const float2* khn = khalf + (size_t)d.src_k[kk + 1] * N * HALF;
+#pragma unroll
+      for (int i = 0; i < E; ++i) nk[i] = khn[(size_t)out_pos<L, E, false>(t, i) * HALF + x];
     }
     if (edge) {
       __syncthreads();
@@ -142,13 +153,17 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
 #pragma unroll
     for (int i = 0; i < E; ++i) acc[i] = make_float2(0.f, 0.f);
     const int kb = d.dst_off[c], ke = d.dst_off[c + 1];
+    const float2* uhat = reinterpret_cast<const float2*>(d.uhat);
+    // Each kernel's row is prefetched one kernel ahead (loads in flight during the transform).
+    float2 xp[E], xq[E];
+    if (kb < ke) c2r_load<N, E>(uhat + ((size_t)d.dst_k[kb] * N + y) * L, t0, xp, xq);
     for (int kk = kb; kk < ke; ++kk) {
       const int k = d.dst_k[kk];
       const int t = launder(t0);
       const float mu = d.mu[k], is2 = d.inv_s2[k], wk = d.w[k];
-      const float2* X = reinterpret_cast<const float2*>(d.uhat) + ((size_t)k * N + y) * L;
       float2 v[E];
-      c2r_start<N, E>(X, t, tw, v);
+      c2r_form<N, E>(xp, xq, t, tw, v);
+      if (kk + 1 < ke) c2r_load<N, E>(uhat + ((size_t)d.dst_k[kk + 1] * N + y) * L, t, xp, xq);
       fft_run<L, E, ROWS, false, true, kTwN>(v, lds, t, lane, tw);
 #pragma unroll
       for (int i = 0; i < E; ++i) {
@@ -180,12 +195,14 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
 // Mass conservation in FP32.  Exactly, sum_p new_p = sum_n (s_n / Z_n) * sum_{p in N(n)} A_p
 // = sum_n s_n.  Plain FP32 breaks the identity by a systematic ~5e-8 per step (the reference's
 // f32 CPU step drifts the same way): Z_n is not exactly the sum of the A_p that later receive
-// q_n, and in smooth regions these roundings repeat with the same sign.  Here Z is an
-// error-free (hi + lo) sum (TwoSum, horizontal 3-sums then vertical), each site keeps the exact
-// residual r_n = s_n - q_n*(Z_hi + Z_lo) of its division (FMA) and adds it back to its own
-// output.  What is left are the roundings of the gather sums and outputs (no systematic sign).
+// q_n, and in smooth regions these roundings repeat with the same sign.  Here Z and the gather
+// use error-free TwoSum chains (hi + lo) in the reference's summation order (the high part is
+// the reference's plain sum), each site keeps the exact residual r_n = s_n - q_n*(Z_hi + Z_lo)
+// of its division (FMA) and adds it back to its own output.  What is left is the final
+// rounding of each output (no systematic sign).  Measured on the benchmark creature: FP64 mass
+// drift 4e-10 over 300 steps at 2048^2 (plain FP32: 1.5e-5).
 // ---------------------------------------------------------------------------------------
-constexpr int kDiffTW = 64, kDiffThreads = 256;
+constexpr int kDiffTW = 64, kDiffTH = 16, kDiffThreads = 256;
 
 __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
   s = a + b;
@@ -193,81 +210,63 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
   e = (a - (s - bb)) + (b - bb);
 }
 
-// Maps a linear block index onto (tile_x, tile_y) so that each XCD (blocks are dealt to the 8
-// XCDs round-robin) works on a contiguous row-major run of tiles: neighbouring tiles share the
-// cache lines of their halo columns, which then come from the same L2.
-__device__ __forceinline__ void xcd_tile2d(int gx, int gy, int& tx, int& ty) {
-  const int lin = blockIdx.x + gx * blockIdx.y, total = gx * gy;
-  int m = lin;
-  if (total % 8 == 0) m = (lin % 8) * (total / 8) + lin / 8;
-  tx = m % gx;
-  ty = m / gx;
-}
-
-template <int TH>
 __global__ void __launch_bounds__(kDiffThreads)
 k_diffusion(const CreatureDev* __restrict__ descs, int sin, int sout, int N) {
-  constexpr int TW = kDiffTW;
+  constexpr int TW = kDiffTW, TH = kDiffTH;
   const CreatureDev& d = descs[blockIdx.z];
   if (d.N != N) return;
   const int tiles_y = (N + TH - 1) / TH;
-  int bx, by;
-  xcd_tile2d(gridDim.x, gridDim.y, bx, by);
-  const int c = by / tiles_y;
+  const int c = blockIdx.y / tiles_y;
   if (c >= d.C) return;
-  const int y0 = (by % tiles_y) * TH;
-  const int x0 = bx * TW;
+  const int y0 = (blockIdx.y % tiles_y) * TH;
+  const int x0 = blockIdx.x * TW;
   const int mask = N - 1;
   const size_t plane = (size_t)N * N;
   const float* A = d.A + c * plane;
   const float* s = slot_ptr(d, sin) + c * plane;
   float* o = slot_ptr(d, sout) + c * plane;
 
-  __shared__ float sA[TH + 4][TW + 5];   // A, rows y0-2.., cols x0-2..
-  __shared__ float sHh[TH + 4][TW + 3];  // horizontal 3-sums of A (hi), cols x0-1..
-  __shared__ float sHl[TH + 4][TW + 3];  //   and their low parts
-  __shared__ float sQ[TH + 2][TW + 3];   // q = s / Z, rows y0-1.., cols x0-1..
-  __shared__ float sR[TH][TW + 1];       // division residuals of the interior
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4 threads
-
-  for (int r = ty; r < TH + 4; r += 4) {
-    const size_t row = (size_t)((y0 - 2 + r) & mask) * N;
-    sA[r][tx] = A[row + ((x0 - 2 + tx) & mask)];
-    if (tx < 4) sA[r][tx + 64] = A[row + ((x0 + 62 + tx) & mask)];
+  __shared__ float sA[TH + 4][TW + 5];
+  __shared__ float sQ[TH + 2][TW + 3];
+  __shared__ float sR[TH + 2][TW + 3];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < (TH + 4) * (TW + 4); i += kDiffThreads) {
+    const int r = i / (TW + 4), cc = i % (TW + 4);
+    const int gy = (y0 - 2 + r) & mask, gx = (x0 - 2 + cc) & mask;
+    sA[r][cc] = A[(size_t)gy * N + gx];
   }
   __syncthreads();
-  for (int r = ty; r < TH + 4; r += 4) {
-    for (int cc = tx; cc < TW + 2; cc += 64) {
-      float h, e1, e2;
-      two_sum(sA[r][cc], sA[r][cc + 1], h, e1);
-      two_sum(h, sA[r][cc + 2], h, e2);
-      sHh[r][cc] = h;
-      sHl[r][cc] = e1 + e2;
-    }
-  }
-  __syncthreads();
-  for (int r = ty; r < TH + 2; r += 4) {
-    const size_t row = (size_t)((y0 - 1 + r) & mask) * N;
-    for (int cc = tx; cc < TW + 2; cc += 64) {
-      float zh, e1, e2;
-      two_sum(sHh[r][cc], sHh[r + 1][cc], zh, e1);
-      two_sum(zh, sHh[r + 2][cc], zh, e2);
-      const float zl = (e1 + e2) + (sHl[r][cc] + sHl[r + 1][cc] + sHl[r + 2][cc]);
-      const float sv = s[row + ((x0 - 1 + cc) & mask)];
-      const float q = sv / zh;
-      sQ[r][cc] = q;
-      if (r >= 1 && r <= TH && cc >= 1 && cc <= TW) sR[r - 1][cc - 1] = fmaf(-q, zl, fmaf(-q, zh, sv));
-    }
-  }
-  __syncthreads();
-  for (int r = ty; r < TH; r += 4) {
-    const int gy = y0 + r, gx = x0 + tx;
-    float acc = 0.f;
+  for (int i = tid; i < (TH + 2) * (TW + 2); i += kDiffThreads) {
+    const int r = i / (TW + 2), cc = i % (TW + 2);
+    float zh = sA[r][cc], zl = 0.f;
 #pragma unroll
-    for (int dr = 0; dr < 3; ++dr)
+    for (int j = 1; j < 9; ++j) {
+      float e;
+      two_sum(zh, sA[r + j / 3][cc + j % 3], zh, e);
+      zl += e;
+    }
+    const int gy = (y0 - 1 + r) & mask, gx = (x0 - 1 + cc) & mask;
+    const float sv = s[(size_t)gy * N + gx];
+    const float q = sv / zh;
+    sQ[r][cc] = q;
+    sR[r][cc] = fmaf(-q, zl, fmaf(-q, zh, sv));  // s - q*(zh + zl)
+  }
+  __syncthreads();
+  for (int i = tid; i < TH * TW; i += kDiffThreads) {
+    const int r = i / TW, cc = i % TW;
+    const int gy = y0 + r, gx = x0 + cc;
+    float sh = sQ[r][cc], sl = 0.f;
 #pragma unroll
-      for (int dc = 0; dc < 3; ++dc) acc += sQ[r + dr][tx + dc];
-    if (gy < N && gx < N) o[(size_t)gy * N + gx] = fmaf(sA[r + 2][tx + 2], acc, sR[r][tx]);
+    for (int j = 1; j < 9; ++j) {
+      float e;
+      two_sum(sh, sQ[r + j / 3][cc + j % 3], sh, e);
+      sl += e;
+    }
+    const float a = sA[r + 2][cc + 2];
+    const float t = a * sh;
+    const float err = fmaf(a, sh, -t);
+    const float v = t + (err + fmaf(a, sl, sR[r + 1][cc + 1]));
+    if (gy < N && gx < N) o[(size_t)gy * N + gx] = v;
   }
 }
 
@@ -380,15 +379,9 @@ static hipError_t launch_step_n(const StepLaunch& sl, hipStream_t st) {
   if ((e = launch_rows_e<N>(sl, st, false)) != hipSuccess) return e;
   if ((e = record(sl.ev[3], st)) != hipSuccess) return e;
   if (!sl.skip_diffusion) {
-    const int th = sl.diff_th ? sl.diff_th : 16;
-    const int tiles_x = (N + kDiffTW - 1) / kDiffTW, tiles_y = (N + th - 1) / th;
+    const int tiles_x = (N + kDiffTW - 1) / kDiffTW, tiles_y = (N + kDiffTH - 1) / kDiffTH;
     const dim3 g_dif(tiles_x, tiles_y * sl.max_c, sl.count);
-    if (th == 32)
-      k_diffusion<32><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N);
-    else if (th == 16)
-      k_diffusion<16><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N);
-    else
-      return hipErrorInvalidValue;
+    k_diffusion<<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N);
   }
   if ((e = record(sl.ev[4], st)) != hipSuccess) return e;
   return hipGetLastError();
