@@ -51,6 +51,110 @@ def cpu_baseline(sample_steps):
                       f"1 thread, {dt:.1f} s)"}
 
 
+def _adam(p, g, m, v, t, lr=5e-3, b1=0.9, b2=0.999, eps=1e-8):
+    m[:] = b1 * m + (1 - b1) * g
+    v[:] = b2 * v + (1 - b2) * g * g
+    return p - lr * (m / (1 - b1 ** t)) / (np.sqrt(v / (1 - b2 ** t)) + eps)
+
+
+def _sanitize(mu, sigma, w):
+    """Parameter box of the reference's LeniaParams._sanitize (train/utils/leniaparams.py:351-360)."""
+    return np.clip(mu, 0.0, 2.0), np.maximum(sigma, 1e-3), np.maximum(w, 0.0)
+
+
+def batched_grad(pkg, count=32, n=512, C=5, T=32, ckpt=4, iters=50, plain_iters=None, seed=0):
+    """Batched gradient-based creature search: `count` creatures (C channels, K = C^2 kernels,
+    radii U{10..18}), T-step rollouts with a checkpoint every `ckpt` steps, MSE to the state a
+    second seeded creature reaches after T steps, Adam on mu/sigma/weights.  Rate in
+    grad-steps/s (one creature advanced and back-propagated through one step), batched (one
+    engine, one graph per iteration for all creatures) vs the plain per-creature loop."""
+    rng = np.random.default_rng(seed)
+    radii = rng.integers(10, 19, size=count)
+    crs = [pkg.baseline_creature(n, num_channels=C, radius=int(radii[i]), seed=seed, creature=i) for i in range(count)]
+    tgs = [pkg.baseline_creature(n, num_channels=C, radius=int(radii[i]), seed=seed, creature=1000 + i)
+           for i in range(count)]
+    tb = pkg.CreatureBatch(tgs)
+    for i, cr in enumerate(tgs):
+        tb.load(i, cr)
+    tb.run_steps(T)
+    targets = [tb.creature_download(i) for i in range(count)]
+    tb.close()
+
+    def run(engines, idx, n_iter):
+        params = [[crs[i]["mu"].astype(np.float64), crs[i]["sigma"].astype(np.float64),
+                   crs[i]["weights"].astype(np.float64)] for i in range(count)]
+        mom = [[np.zeros_like(p) for p in ps] for ps in params]
+        vel = [[np.zeros_like(p) for p in ps] for ps in params]
+        losses = []
+        t0 = None
+        for it in range(n_iter + 1):  # iteration 0 captures the graphs (untimed)
+            if it == 1:
+                t0 = time.perf_counter()
+            for eng, members in engines:
+                eng.rollout_grad(T, ckpt)
+                for j, i in enumerate(members):
+                    g = eng.creature_grads(j, with_state=False)
+                    new = [_adam(params[i][q], g[key].astype(np.float64), mom[i][q], vel[i][q], it + 1)
+                           for q, key in enumerate(("mu", "sigma", "weights"))]
+                    params[i] = list(_sanitize(*new))
+                    eng.creature_set_growth(j, *params[i])
+                if it == n_iter:
+                    losses += [eng.creature_loss(j) for j in range(len(members))]
+        return (time.perf_counter() - t0), losses
+
+    def build(members):
+        eng = pkg.CreatureBatch([crs[i] for i in members])
+        for j, i in enumerate(members):
+            eng.load(j, crs[i])
+            eng.creature_set_target(j, targets[i])
+        return eng, members
+
+    batch = [build(list(range(count)))]
+    dt_b, loss_b = run(batch, None, iters)
+    for e, _ in batch:
+        e.close()
+    plain = [build([i]) for i in range(count)]
+    pi = plain_iters or iters
+    dt_p, loss_p = run(plain, None, pi)
+    for e, _ in plain:
+        e.close()
+    rate_b = count * T * iters / dt_b
+    rate_p = count * T * pi / dt_p
+    return {"config": f"{count} creatures {n}x{n} C{C}K{C * C} R~U{{10..18}}, T={T}, ckpt {ckpt}, Adam on "
+                      f"mu/sigma/w, {iters} iterations (plain loop: {pi})",
+            "grad_steps_per_s_batched": round(rate_b, 1), "grad_steps_per_s_plain": round(rate_p, 1),
+            "ratio": round(rate_b / rate_p, 2), "final_mse_mean_batched": float(np.mean(loss_b)),
+            "final_mse_mean_plain": float(np.mean(loss_p))}
+
+
+def batched_forward(pkg, steps=200, seed=0):
+    """256 creatures of uneven sizes (85 at 256^2, 85 at 512^2, 86 at 1024^2), C ~ U{2,3,4},
+    K = C^2, radii U{8..20}, one engine and one graph per step for all of them."""
+    rng = np.random.default_rng(seed)
+    sizes = [256] * 85 + [512] * 85 + [1024] * 86
+    crs = []
+    for i, n in enumerate(sizes):
+        C = int(rng.integers(2, 5))
+        crs.append(pkg.baseline_creature(n, num_channels=C, radius=int(rng.integers(8, 21)), seed=seed, creature=i))
+    b = pkg.CreatureBatch(crs)
+    for i, cr in enumerate(crs):
+        b.load(i, cr)
+    plans = b.autotune(3)
+    b.run_steps(5)
+    b.sync()
+    t0 = time.perf_counter()
+    b.run_steps(steps)
+    b.sync()
+    dt = time.perf_counter() - t0
+    b.close()
+    cells = sum(n * n for n in sizes)
+    alg = sum((28 * cr["num_channels"] + 12 * cr["num_kernels"]) * cr["n"] ** 2 for cr in crs)
+    return {"config": f"{len(crs)} creatures (85x256^2, 85x512^2, 86x1024^2), C~U{{2,3,4}}, K=C^2, R~U{{8..20}}, "
+                      f"{steps} steps", "cell_steps_per_s": round(cells * steps / dt, 1),
+            "ms_per_step": round(dt * 1e3 / steps, 4), "alg_GBps": round(alg * steps / dt / 1e9, 1),
+            "hbm_frac": round(alg * steps / dt / 1e9 / HBM_PEAK_GBS, 4), "plans": plans}
+
+
 def load_traffic(workload_tag, kernel):
     """Measured HBM bytes per launch from committed rocprofv3 PMC summaries, if present."""
     path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload_tag}.json")
@@ -69,6 +173,11 @@ def main():
     ap.add_argument("--radius", type=int, default=26)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="step2048", choices=["step2048", "batched-forward", "batched-grad"],
+                    help="step2048: the headline 2048^2 C3K9 step; the batched workloads print their own line")
+    ap.add_argument("--iters", type=int, default=50, help="batched-grad: Adam iterations")
+    ap.add_argument("--plain-iters", type=int, default=0, help="batched-grad: iterations of the plain loop")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a device-to-device copy of this many MiB (PMC calibration only)")
     args = ap.parse_args()
@@ -78,6 +187,14 @@ def main():
         sys.exit(2)
 
     pkg = ge.load_package()
+    if args.workload == "batched-grad":
+        print(json.dumps({"workload": "batched-grad", **batched_grad(pkg, iters=args.iters,
+                                                                      plain_iters=args.plain_iters or None)}),
+              flush=True)
+        return
+    if args.workload == "batched-forward":
+        print(json.dumps({"workload": "batched-forward", **batched_forward(pkg, steps=args.steps)}), flush=True)
+        return
     if args.copy_calibration_mb:
         import torch
         a = torch.ones(args.copy_calibration_mb << 18, dtype=torch.float32, device="cuda:0")
@@ -89,6 +206,7 @@ def main():
     cr = pkg.baseline_creature(n, num_channels=C, radius=R, seed=0)
     K = cr["num_kernels"]
     eng = pkg.make_engine(cr)
+    plan = eng.plans()[0] if args.no_autotune else eng.autotune(5)[0]
     try:
         import torch
         sync_dev = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
@@ -130,7 +248,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"DiffusionLenia {n}x{n} C{C}K{K} R={R} periodic, temp 1, seed 0",
-                   "grid": n, "channels": C, "kernels": K, "radius": R},
+                   "grid": n, "channels": C, "kernels": K, "radius": R, "plan": plan},
         "roofline": roof,
         "kernels": per_kernel,
         "step_roofline": {"alg_bytes_per_step": step_alg, "GBps": round(step_alg / (ms_per_step * 1e-3) / 1e9, 1),

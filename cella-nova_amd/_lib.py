@@ -58,6 +58,11 @@ class CreatureParams(ctypes.Structure):
     ]
 
 
+class Plan(ctypes.Structure):
+    _fields_ = [("n", c_uint32), ("fwd_e", c_uint32), ("inv_e", c_uint32), ("col_e", c_uint32),
+                ("col_tc", c_uint32), ("diff_th", c_uint32)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/cellanova.h
 SIGNATURES = {
     "cn_engine_create": (c_int, [POINTER(EngineDesc), POINTER(c_void_p)]),
@@ -81,6 +86,11 @@ SIGNATURES = {
     "cn_generate_creature": (c_int, [POINTER(CreatureParams), c_void_p, c_void_p, c_void_p, c_void_p,
                                      c_void_p, c_void_p]),
     "cn_debug_fail_next_hip": (None, [c_int]),
+    # plans / autotuner
+    "cn_autotune": (c_int, [c_void_p, c_uint32]),
+    "cn_engine_size_classes": (c_uint32, [c_void_p]),
+    "cn_engine_get_plan": (c_int, [c_void_p, c_uint32, POINTER(Plan)]),
+    "cn_engine_set_plan": (c_int, [c_void_p, POINTER(Plan)]),
     # batches
     "cn_batch_create": (c_int, [POINTER(EngineDesc), c_uint32, c_int, POINTER(c_void_p)]),
     "cn_creature_set_kernel": (c_int, [c_void_p, c_uint32, c_uint32, c_void_p, c_size_t]),

@@ -360,8 +360,8 @@ static hipError_t launch_adjoint_n(const StepLaunch& sl, int gin, int gout, hipS
   if (e != hipSuccess) return e;
   const int tiles_x = (N + kAdjTW - 1) / kAdjTW, tiles_y = (N + kAdjTH - 1) / kAdjTH;
   k_diffusion_adj<<<dim3(tiles_x, tiles_y * sl.max_c, sl.count), kAdjThreads, 0, st>>>(sl.descs, sl.sin, gin, N);
-  const int re = sl.row_e ? sl.row_e : DefaultGeom<N>::ROW_E;
   auto rows = [&](bool growth) {
+    const int re = growth ? (sl.fwd_e ? sl.fwd_e : DefaultGeom<N>::FWD_E) : (sl.inv_e ? sl.inv_e : DefaultGeom<N>::INV_E);
     switch (re) {
       case 8: return launch_adj_rows<N, 8>(sl, gout, st, growth);
       case 16: return launch_adj_rows<N, 16>(sl, gout, st, growth);

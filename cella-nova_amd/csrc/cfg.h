@@ -24,8 +24,10 @@ struct ColCfg {
 // Built-in launch geometry (used when no tuned plan is given).
 template <int N>
 struct DefaultGeom {
-  static constexpr int ROW_E = 8;
+  static constexpr int FWD_E = 8;
+  static constexpr int INV_E = 8;
   static constexpr int COL_E = 16;
+  static constexpr int DIFF_TH = 16;
   static constexpr int TC_DEF = N >= 2048 ? 4 : (N >= 1024 ? 8 : 16);
   static constexpr int TC = TC_DEF < N / 2 ? TC_DEF : N / 2;
 };

@@ -169,7 +169,7 @@ struct SizeClass {
   std::vector<int> members;
   CreatureDev* d_descs = nullptr;
   int max_c = 0, max_k = 0;
-  int row_e = 0, col_e = 0, col_tc = 0;  // launch geometry (0: built-in default)
+  int fwd_e = 0, inv_e = 0, col_e = 0, col_tc = 0, diff_th = 0;  // launch geometry (0: default)
 };
 
 struct Rollout {
@@ -255,9 +255,11 @@ StepLaunch class_launch(cn_engine* e, const SizeClass& k, int sin, int sout) {
   sl.max_c = k.max_c;
   sl.max_k = k.max_k;
   sl.tw = e->tw;
-  sl.row_e = k.row_e;
+  sl.fwd_e = k.fwd_e;
+  sl.inv_e = k.inv_e;
   sl.col_e = k.col_e;
   sl.col_tc = k.col_tc;
+  sl.diff_th = k.diff_th;
   return sl;
 }
 
@@ -517,10 +519,7 @@ cn_status create_engine(const cn_engine_desc* descs, uint32_t count, int device,
       e->cls[ci].N = (int)d.n;
     }
     SizeClass& k = e->cls[ci];
-    // Geometry overrides for experiments: CN_ROW_E, CN_COL_E, CN_COL_TC (all size classes).
-    if (const char* v = std::getenv("CN_ROW_E")) k.row_e = std::atoi(v);
-    if (const char* v = std::getenv("CN_COL_E")) k.col_e = std::atoi(v);
-    if (const char* v = std::getenv("CN_COL_TC")) k.col_tc = std::atoi(v);
+
     c.cls = ci;
     c.idx = (int)k.members.size();
     k.members.push_back((int)i);
@@ -996,6 +995,149 @@ cn_status cn_creature_get_grads(cn_engine* e, uint32_t i, float* grad_state0, si
   for (int j = 0; j < 3; ++j)
     if (outs[j]) CN_HIP(hipMemcpyAsync(outs[j], c.dev.gparam + (size_t)j * c.K, c.K * 4, kind, e->stream));
   CN_HIP(hipStreamSynchronize(e->stream));
+  return CN_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// autotuner: picks, per size class, the FFT geometry (values per thread of the forward and
+// inverse row transforms and of the column transform, column tile width) and the diffusion
+// tile height by timing every candidate on the engine's own buffers with HIP events.
+// ---------------------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+bool col_geom_ok(int N, int ce, int tc) {
+  const int E = std::min(ce, N), T = N / E;
+  return tc <= N / 2 && tc * T <= 1024 && (long)(N + N / 16) * tc * 8 <= 160L * 1024;
+}
+
+struct Geom {
+  int fwd_e, inv_e, col_e, col_tc, diff_th;
+};
+
+// Launches one step of size class k with geometry g and events around its kernels; returns
+// the four kernel times.  The step reads the current slot and writes the other one (scratch
+// here: the current state is not advanced).
+cn_status time_class_step(cn_engine* e, SizeClass& k, const Geom& g, float ms[4]) {
+  StepLaunch sl = class_launch(e, k, e->cur, e->cur ^ 1);
+  sl.fwd_e = g.fwd_e;
+  sl.inv_e = g.inv_e;
+  sl.col_e = g.col_e;
+  sl.col_tc = g.col_tc;
+  sl.diff_th = g.diff_th;
+  for (int q = 0; q < 5; ++q) sl.ev[q] = e->ev[q];
+  CN_HIP(launch_step(k.N, sl, e->stream));
+  CN_HIP(hipEventSynchronize(e->ev[4]));
+  for (int q = 0; q < 4; ++q) CN_HIP(hipEventElapsedTime(&ms[q], e->ev[q], e->ev[q + 1]));
+  return CN_OK;
+}
+
+float median(std::vector<float> v) {
+  std::sort(v.begin(), v.end());
+  return v.empty() ? 1e30f : v[v.size() / 2];
+}
+
+cn_status tune_class(cn_engine* e, SizeClass& k, uint32_t reps) {
+  const int N = k.N;
+  const std::vector<int> rows = {8, 16};
+  std::vector<std::pair<int, int>> cols;
+  for (int ce : {8, 16})
+    for (int tc : {2, 4, 8, 16})
+      if (col_geom_ok(N, ce, tc)) cols.push_back({ce, tc});
+  const std::vector<int> diffs = {8, 16, 32};
+  std::vector<std::vector<float>> tf(rows.size()), ti(rows.size()), tcol(cols.size()), td(diffs.size());
+  const size_t rounds = std::max({rows.size(), cols.size(), diffs.size()});
+  // one untimed pass per candidate set to load code objects and warm caches
+  for (size_t r = 0; r < rounds * (reps + 1); ++r) {
+    const size_t i = r % rounds;
+    const Geom g{rows[i % rows.size()], rows[(i + 1) % rows.size()], cols[i % cols.size()].first,
+                 cols[i % cols.size()].second, diffs[i % diffs.size()]};
+    float ms[4];
+    const cn_status st = time_class_step(e, k, g, ms);
+    if (st != CN_OK) return st;
+    if (r < rounds) continue;
+    tf[i % rows.size()].push_back(ms[0]);
+    tcol[i % cols.size()].push_back(ms[1]);
+    ti[(i + 1) % rows.size()].push_back(ms[2]);
+    td[i % diffs.size()].push_back(ms[3]);
+  }
+  auto best = [](const std::vector<std::vector<float>>& t) {
+    size_t b = 0;
+    for (size_t j = 1; j < t.size(); ++j)
+      if (median(t[j]) < median(t[b])) b = j;
+    return b;
+  };
+  k.fwd_e = rows[best(tf)];
+  k.inv_e = rows[best(ti)];
+  k.col_e = cols[best(tcol)].first;
+  k.col_tc = cols[best(tcol)].second;
+  k.diff_th = diffs[best(td)];
+  return CN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+cn_status cn_autotune(cn_engine* e, uint32_t reps) {
+  clear_error();
+  if (!e) return set_error(CN_ERR_INVALID_ARG, "null engine");
+  if (reps == 0 || reps > 1000) return set_error(CN_ERR_INVALID_ARG, "reps must be in [1, 1000]");
+  CN_HIP(hipSetDevice(e->device));
+  CN_HIP(hipStreamSynchronize(e->stream));
+  for (SizeClass& k : e->cls) {
+    const cn_status st = tune_class(e, k, reps);
+    if (st != CN_OK) return st;
+  }
+  drop_graphs(e);        // graphs bake the launch geometry in
+  e->have_prev = false;  // the other slot was used as scratch
+  e->ro_fwd_done = false;
+  return CN_OK;
+}
+
+uint32_t cn_engine_size_classes(const cn_engine* e) { return e ? (uint32_t)e->cls.size() : 0; }
+
+cn_status cn_engine_get_plan(const cn_engine* e, uint32_t size_class, cn_plan* out) {
+  clear_error();
+  if (!e || !out) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  if (size_class >= e->cls.size()) return set_error(CN_ERR_INVALID_ARG, "size class %u out of range", size_class);
+  const SizeClass& k = e->cls[size_class];
+  out->n = (uint32_t)k.N;
+  out->fwd_e = (uint32_t)k.fwd_e;
+  out->inv_e = (uint32_t)k.inv_e;
+  out->col_e = (uint32_t)k.col_e;
+  out->col_tc = (uint32_t)k.col_tc;
+  out->diff_th = (uint32_t)k.diff_th;
+  return CN_OK;
+}
+
+cn_status cn_engine_set_plan(cn_engine* e, const cn_plan* plan) {
+  clear_error();
+  if (!e || !plan) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  SizeClass* k = nullptr;
+  for (SizeClass& c : e->cls)
+    if (c.N == (int)plan->n) k = &c;
+  if (!k) return set_error(CN_ERR_INVALID_ARG, "no creature with grid size %u", plan->n);
+  auto row_ok = [](uint32_t v) { return v == 0 || v == 8 || v == 16; };
+  if (!row_ok(plan->fwd_e) || !row_ok(plan->inv_e) || !row_ok(plan->col_e))
+    return set_error(CN_ERR_INVALID_ARG, "values per thread must be 0 (default), 8 or 16");
+  if (plan->diff_th != 0 && plan->diff_th != 8 && plan->diff_th != 16 && plan->diff_th != 32)
+    return set_error(CN_ERR_INVALID_ARG, "diffusion tile height must be 0, 8, 16 or 32");
+  if (plan->col_tc != 0 || plan->col_e != 0) {
+    const int ce = plan->col_e ? (int)plan->col_e : 16;
+    const int tc = plan->col_tc ? (int)plan->col_tc : 4;
+    if (!col_geom_ok(k->N, ce, tc))
+      return set_error(CN_ERR_INVALID_ARG, "column geometry (E=%d, TC=%d) does not fit N=%d", ce, tc, k->N);
+  }
+  k->fwd_e = (int)plan->fwd_e;
+  k->inv_e = (int)plan->inv_e;
+  k->col_e = (int)plan->col_e;
+  k->col_tc = (int)plan->col_tc;
+  k->diff_th = (int)plan->diff_th;
+  CN_HIP(hipSetDevice(e->device));
+  CN_HIP(hipStreamSynchronize(e->stream));
+  drop_graphs(e);
   return CN_OK;
 }
 

@@ -202,7 +202,7 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
 // rounding of each output (no systematic sign).  Measured on the benchmark creature: FP64 mass
 // drift 4e-10 over 300 steps at 2048^2 (plain FP32: 1.5e-5).
 // ---------------------------------------------------------------------------------------
-constexpr int kDiffTW = 64, kDiffTH = 16, kDiffThreads = 256;
+constexpr int kDiffTW = 64, kDiffThreads = 256;
 
 __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
   s = a + b;
@@ -210,9 +210,10 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
   e = (a - (s - bb)) + (b - bb);
 }
 
+template <int TH>
 __global__ void __launch_bounds__(kDiffThreads)
 k_diffusion(const CreatureDev* __restrict__ descs, int sin, int sout, int N) {
-  constexpr int TW = kDiffTW, TH = kDiffTH;
+  constexpr int TW = kDiffTW;
   const CreatureDev& d = descs[blockIdx.z];
   if (d.N != N) return;
   const int tiles_y = (N + TH - 1) / TH;
@@ -355,7 +356,8 @@ static hipError_t launch_rows(const StepLaunch& sl, hipStream_t st, bool fwd) {
 
 template <int N>
 static hipError_t launch_rows_e(const StepLaunch& sl, hipStream_t st, bool fwd) {
-  switch (sl.row_e ? sl.row_e : DefaultGeom<N>::ROW_E) {
+  const int e = fwd ? (sl.fwd_e ? sl.fwd_e : DefaultGeom<N>::FWD_E) : (sl.inv_e ? sl.inv_e : DefaultGeom<N>::INV_E);
+  switch (e) {
     case 8: return launch_rows<N, 8>(sl, st, fwd);
     case 16: return launch_rows<N, 16>(sl, st, fwd);
     default: return hipErrorInvalidValue;
@@ -379,9 +381,15 @@ static hipError_t launch_step_n(const StepLaunch& sl, hipStream_t st) {
   if ((e = launch_rows_e<N>(sl, st, false)) != hipSuccess) return e;
   if ((e = record(sl.ev[3], st)) != hipSuccess) return e;
   if (!sl.skip_diffusion) {
-    const int tiles_x = (N + kDiffTW - 1) / kDiffTW, tiles_y = (N + kDiffTH - 1) / kDiffTH;
+    const int th = sl.diff_th ? sl.diff_th : DefaultGeom<N>::DIFF_TH;
+    const int tiles_x = (N + kDiffTW - 1) / kDiffTW, tiles_y = (N + th - 1) / th;
     const dim3 g_dif(tiles_x, tiles_y * sl.max_c, sl.count);
-    k_diffusion<<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N);
+    switch (th) {
+      case 8: k_diffusion<8><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N); break;
+      case 16: k_diffusion<16><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N); break;
+      case 32: k_diffusion<32><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N); break;
+      default: return hipErrorInvalidValue;
+    }
   }
   if ((e = record(sl.ev[4], st)) != hipSuccess) return e;
   return hipGetLastError();

@@ -5,7 +5,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import EngineDesc, EngineInfo, check, lib
+from ._lib import EngineDesc, EngineInfo, Plan, check, lib
 
 
 def _u32(a):
@@ -126,12 +126,12 @@ class _Engine:
         check(lib().cn_creature_get_loss(self._h, int(i), ctypes.byref(v)))
         return v.value
 
-    def creature_grads(self, i, state=None, mu=None, sigma=None, w=None, device=False):
-        """Gradients of the last backward.  Host: returns numpy arrays.  device=True: fills the
-        given device tensors."""
+    def creature_grads(self, i, state=None, mu=None, sigma=None, w=None, device=False, with_state=True):
+        """Gradients of the last backward.  Host: returns numpy arrays (dL/dstate0 only if
+        with_state).  device=True: fills the given device tensors."""
         C, n, K = self._shapes[i]
         if not device:
-            state = np.empty(C * n * n, np.float32)
+            state = np.empty(C * n * n, np.float32) if with_state else None
             mu, sigma, w = (np.empty(K, np.float32) for _ in range(3))
         check(lib().cn_creature_get_grads(
             self._h, int(i), _ptr(state) if state is not None else None, C * n * n,
@@ -160,6 +160,23 @@ class _Engine:
         """dL/dx_T per creature as contiguous f32 device tensors."""
         arr = (ctypes.c_void_p * len(grad_finals))(*[g.data_ptr() for g in grad_finals])
         check(lib().cn_rollout_backward(self._h, arr, len(grad_finals)))
+
+    def autotune(self, reps=5):
+        """Pick the fastest FFT / stencil geometry per grid size on the device."""
+        check(lib().cn_autotune(self._h, int(reps)))
+        return self.plans()
+
+    def plans(self):
+        out = []
+        for i in range(lib().cn_engine_size_classes(self._h)):
+            p = Plan()
+            check(lib().cn_engine_get_plan(self._h, i, ctypes.byref(p)))
+            out.append({f: getattr(p, f) for f, _ in Plan._fields_})
+        return out
+
+    def set_plan(self, n, fwd_e=0, inv_e=0, col_e=0, col_tc=0, diff_th=0):
+        p = Plan(int(n), int(fwd_e), int(inv_e), int(col_e), int(col_tc), int(diff_th))
+        check(lib().cn_engine_set_plan(self._h, ctypes.byref(p)))
 
     def profile_kernels(self, steps):
         """Average device ms per launch over `steps` directly launched steps (first size class):

@@ -161,6 +161,28 @@ cn_status cn_creature_get_grads(cn_engine* e, uint32_t creature, float* grad_sta
                                 float* grad_sigma, float* grad_w, size_t K, int device_pointers);
 
 /* ------------------------------------------------------------------------------------ */
+/* Launch plans and the autotuner (no reference counterpart: the reference hard-codes one */
+/* shader per grid size, mod.rs:30-39)                                                    */
+/* ------------------------------------------------------------------------------------ */
+
+/* Geometry of one grid-size class: values per thread (8 or 16) of the forward row (R2C),
+ * inverse row (C2R) and column FFTs, column tile width of the column passes (2..16), and the
+ * diffusion tile height (8, 16, 32).  0 selects the built-in default. */
+typedef struct cn_plan {
+  uint32_t n;
+  uint32_t fwd_e, inv_e, col_e, col_tc, diff_th;
+} cn_plan;
+
+/* Times every candidate geometry of every size class on the device (reps timed launches per
+ * candidate, median) and keeps the fastest per kernel.  Synchronises; the current state is
+ * unchanged.  Captured graphs are rebuilt on next use. */
+cn_status cn_autotune(cn_engine* e, uint32_t reps);
+uint32_t cn_engine_size_classes(const cn_engine* e);
+cn_status cn_engine_get_plan(const cn_engine* e, uint32_t size_class, cn_plan* out);
+/* Applies a plan to the size class with grid size plan->n (validated; graphs rebuilt). */
+cn_status cn_engine_set_plan(cn_engine* e, const cn_plan* plan);
+
+/* ------------------------------------------------------------------------------------ */
 /* Loader (ml-rs/config.rs)                                                              */
 /* ------------------------------------------------------------------------------------ */
 

@@ -14,7 +14,7 @@ HEADER = os.path.join(ROOT, "include", "cellanova.h")
 
 def declared_symbols():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:cn_status|void|const char\*)\s+(cn_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:cn_status|void|const char\*|uint32_t)\s+(cn_\w+)\s*\(", text, re.M)))
 
 
 def test_header_symbols_are_exported_and_bound(cn):
