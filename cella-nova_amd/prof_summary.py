@@ -1,0 +1,90 @@
+"""Summaries of rocprofv3 output for the profiles/ directory.
+
+  python cella-nova_amd/prof_summary.py stats <run_kernel_stats.csv>
+  python cella-nova_amd/prof_summary.py traffic <fetch run_counter_collection.csv> <write ...csv> [calib MiB] [--json DIR TAG]
+
+`traffic` averages FETCH_SIZE / WRITE_SIZE (KiB per dispatch) per kernel and scales them by what
+the same counters read for the calibration copy (a device copy of `calib` MiB, dispatched as
+__amd_rocclr_copyBuffer): factor = expected KiB / measured KiB.  With --json it also writes
+DIR/traffic_<kernel>_<TAG>.json (bytes per launch) for bench.py.
+"""
+import collections
+import csv
+import json
+import os
+import sys
+
+SHORT = {"k_row_fwd": "row_fwd_r2c", "k_col_fused": "col_fused", "k_row_inv_growth": "row_inv_growth",
+         "k_diffusion": "diffusion"}
+
+
+def short(name):
+    for k, v in SHORT.items():
+        if f"cn::{k}" in name:
+            return v
+    return None
+
+
+def variant(name):
+    """Kernel name with its template arguments, e.g. k_col_fused<2048, 4, 16>."""
+    base = name.split("(")[0].replace("void ", "").replace("cn::", "")
+    return base
+
+
+def stats(path):
+    rows = list(csv.DictReader(open(path)))
+    print(f"{'kernel':<72} {'calls':>6} {'avg_us':>9} {'min_us':>9} {'max_us':>9} {'pct':>6}")
+    for r in rows:
+        print(f"{r['Name'][:72]:<72} {int(r['Calls']):>6} {float(r['AverageNs']) / 1e3:>9.2f} "
+              f"{float(r['MinNs']) / 1e3:>9.2f} {float(r['MaxNs']) / 1e3:>9.2f} {float(r['Percentage']):>6.1f}")
+
+
+def _per_kernel(path, counter):
+    agg = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] == counter:
+            agg[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return agg
+
+
+def traffic(fetch_csv, write_csv, calib_mib=256, json_dir=None, tag=None):
+    f, w = _per_kernel(fetch_csv, "FETCH_SIZE"), _per_kernel(write_csv, "WRITE_SIZE")
+    exp = calib_mib * 1024.0
+    cf = [v for v in f.get("__amd_rocclr_copyBuffer", []) if v > 0.25 * exp]
+    cw = [v for v in w.get("__amd_rocclr_copyBuffer", []) if v > 0.5 * exp]
+    ff = exp / (sum(cf) / len(cf)) if cf else 1.0
+    fw = exp / (sum(cw) / len(cw)) if cw else 1.0
+    print(f"calibration copy {calib_mib} MiB: FETCH factor {ff:.3f}, WRITE factor {fw:.3f}")
+    print(f"{'kernel':<44} {'launches':>8} {'fetch_MB':>10} {'write_MB':>10} {'total_MB':>10}")
+    most = {}
+    for name in f:
+        s = short(name)
+        if s and len(f[name]) > most.get(s, (0, None))[0]:
+            most[s] = (len(f[name]), name)
+    for name in f:
+        s = short(name)
+        if not s:
+            continue
+        fb = sum(f[name]) / len(f[name]) * 1024 * ff
+        wb = sum(w[name]) / len(w[name]) * 1024 * fw if name in w else 0.0
+        print(f"{variant(name):<44} {len(f[name]):>8} {fb / 1e6:>10.1f} {wb / 1e6:>10.1f} {(fb + wb) / 1e6:>10.1f}")
+        if json_dir and most[s][1] == name:
+            with open(os.path.join(json_dir, f"traffic_{s}_{tag}.json"), "w") as fh:
+                json.dump({"kernel": s, "variant": variant(name), "workload": tag, "fetch_bytes": round(fb),
+                           "write_bytes": round(wb),
+                           "bytes_per_launch": round(fb + wb), "fetch_factor": round(ff, 4),
+                           "write_factor": round(fw, 4)}, fh)
+                fh.write("\n")
+
+
+if __name__ == "__main__":
+    if sys.argv[1] == "stats":
+        stats(sys.argv[2])
+    else:
+        args = sys.argv[2:]
+        jd = tg = None
+        if "--json" in args:
+            i = args.index("--json")
+            jd, tg = args[i + 1], args[i + 2]
+            args = args[:i]
+        traffic(args[0], args[1], int(args[2]) if len(args) > 2 else 256, jd, tg)
