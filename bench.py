@@ -27,14 +27,16 @@ HBM_PEAK_GBS = 8000.0
 KERNELS = ["row_fwd_r2c", "col_fused", "row_inv_growth", "diffusion"]
 
 
-def algorithmic_bytes_per_cell(C, K):
+def algorithmic_bytes_per_cell(C, K, compact=0):
     """Bytes each kernel must move per grid site (all channels), by its inputs and outputs:
     row_fwd reads C f32 planes, writes C half spectra (N/2 complex per row = 4 B/site);
-    col_fused reads C half spectra + K kernel half spectra, writes K half spectra;
+    col_fused reads C half spectra + K kernel spectra, writes K half spectra; a kernel spectrum
+    is a complex half spectrum (4 B/site) or, for the `compact` kernels whose spectrum is real
+    and even in ky, a real quarter (1 B/site);
     row_inv_growth reads K half spectra, writes C affinity planes;
     diffusion reads C affinity + C state planes, writes C state planes."""
-    return {"row_fwd_r2c": 8 * C, "col_fused": 4 * C + 8 * K, "row_inv_growth": 4 * K + 4 * C,
-            "diffusion": 12 * C}
+    return {"row_fwd_r2c": 8 * C, "col_fused": 4 * C + 4 * K + 4 * (K - compact) + compact,
+            "row_inv_growth": 4 * K + 4 * C, "diffusion": 12 * C}
 
 
 def cpu_baseline(sample_steps):
@@ -230,8 +232,9 @@ def main():
     # Per-kernel average launch durations (HIP events on the engine stream), same number of steps.
     prof_steps = min(args.steps, 200)
     kms = eng.profile_kernels(prof_steps)
+    compact = eng.info()["compact_kernels"]
     eng.close()
-    bpc = algorithmic_bytes_per_cell(C, K)
+    bpc = algorithmic_bytes_per_cell(C, K, compact)
     per_kernel = {}
     for name, ms in zip(KERNELS, kms):
         b = bpc[name] * n * n
@@ -248,7 +251,8 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"DiffusionLenia {n}x{n} C{C}K{K} R={R} periodic, temp 1, seed 0",
-                   "grid": n, "channels": C, "kernels": K, "radius": R, "plan": plan},
+                   "grid": n, "channels": C, "kernels": K, "radius": R, "compact_kernels": compact,
+                   "plan": plan},
         "roofline": roof,
         "kernels": per_kernel,
         "step_roofline": {"alg_bytes_per_step": step_alg, "GBps": round(step_alg / (ms_per_step * 1e-3) / 1e9, 1),

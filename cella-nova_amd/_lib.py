@@ -37,7 +37,7 @@ class EngineDesc(ctypes.Structure):
 
 class EngineInfo(ctypes.Structure):
     _fields_ = [("n", c_uint32), ("num_channels", c_uint32), ("num_kernels", c_uint32),
-                ("steps_done", c_uint64), ("device_bytes", c_uint64)]
+                ("steps_done", c_uint64), ("device_bytes", c_uint64), ("compact_kernels", c_uint32)]
 
 
 class Config(ctypes.Structure):
@@ -78,6 +78,9 @@ SIGNATURES = {
     "cn_download_channels": (c_int, [c_void_p, c_void_p, c_size_t]),
     "cn_download_conv_real": (c_int, [c_void_p, c_void_p, c_size_t]),
     "cn_mass_fp64": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "cn_iterate_and_render": (c_int, [c_void_p]),
+    "cn_download_render": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "cn_creature_download_render": (c_int, [c_void_p, c_uint32, c_void_p, c_size_t]),
     "cn_engine_get_info": (c_int, [c_void_p, POINTER(EngineInfo)]),
     "cn_last_error": (c_char_p, []),
     "cn_load_config": (c_int, [c_char_p, POINTER(POINTER(Config))]),

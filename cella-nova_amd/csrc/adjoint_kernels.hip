@@ -167,6 +167,25 @@ k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) 
   const bool edge = (x0 == 0);
   __shared__ float2 lds[LdsMap<L, TC, true>::size];
 
+  // acc += v * conj(K^_k) at this thread's positions (compact real copy when available)
+  auto accumulate = [&](int k, const float2* v, float2* acc) {
+    if (d.ksym[k]) {
+      const float* kr = d.kreal + (size_t)k * (HALF + 1) * HALF;
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const int y = out_pos<L, E, false>(t, i);
+        const float r = kr[(size_t)(y <= HALF ? y : N - y) * HALF + x];
+        acc[i] = make_float2(acc[i].x + v[i].x * r, acc[i].y + v[i].y * r);
+      }
+    } else {
+      const float2* kh = reinterpret_cast<const float2*>(d.khalf) + (size_t)k * N * HALF;
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const int y = out_pos<L, E, false>(t, i);
+        acc[i] = c_add(acc[i], c_mulc(v[i], kh[(size_t)y * HALF + x]));
+      }
+    }
+  };
   float2 acc[E];
 #pragma unroll
   for (int i = 0; i < E; ++i) acc[i] = make_float2(0.f, 0.f);
@@ -203,18 +222,10 @@ k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) 
           acc[i] = c_add(acc[i], make_float2(pa.x - pb.y, pa.y + pb.x));
         }
       } else {
-#pragma unroll
-        for (int i = 0; i < E; ++i) {
-          const int y = out_pos<L, E, false>(t, i);
-          acc[i] = c_add(acc[i], c_mulc(v[i], kh[(size_t)y * HALF + x]));
-        }
+        accumulate(k, v, acc);
       }
     } else {
-#pragma unroll
-      for (int i = 0; i < E; ++i) {
-        const int y = out_pos<L, E, false>(t, i);
-        acc[i] = c_add(acc[i], c_mulc(v[i], kh[(size_t)y * HALF + x]));
-      }
+      accumulate(k, v, acc);
     }
   }
   fft_run<L, E, TC, true, true, kTwN>(acc, lds, t, lane, tw);

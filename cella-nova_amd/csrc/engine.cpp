@@ -147,6 +147,29 @@ void project_kernel(const float* full, uint32_t N, float* half, float* nyq) {
   }
 }
 
+// True if the (Hermitian-projected) spectrum is real and even in ky to within f32 rounding
+// of its largest coefficient (|Im| and |K(ky) - K(-ky)| <= 2^-23 max|K|), as for the
+// radially symmetric Lenia kernels; then fills `real` with rows ky = 0 .. N/2 of Re K.
+bool real_even_spectrum(const float* half, const float* nyq, uint32_t N, std::vector<float>& real) {
+  const size_t H = N / 2;
+  double mx = 0.0;
+  for (size_t i = 0; i < N * H; ++i) mx = std::max(mx, std::hypot((double)half[2 * i], (double)half[2 * i + 1]));
+  for (size_t y = 0; y < N; ++y) mx = std::max(mx, std::hypot((double)nyq[2 * y], (double)nyq[2 * y + 1]));
+  const double tol = std::ldexp(mx, -23);
+  for (size_t y = 0; y < N; ++y) {
+    const size_t ym = (N - y) & (N - 1);
+    for (size_t x = 0; x < H; ++x) {
+      const float* a = half + 2 * (y * H + x);
+      const float* b = half + 2 * (ym * H + x);
+      if (std::fabs((double)a[1]) > tol || std::fabs((double)a[0] - (double)b[0]) > tol) return false;
+    }
+  }
+  real.resize((H + 1) * H);
+  for (size_t y = 0; y <= H; ++y)
+    for (size_t x = 0; x < H; ++x) real[y * H + x] = half[2 * (y * H + x)];
+  return true;
+}
+
 }  // namespace cn
 
 using namespace cn;
@@ -197,6 +220,7 @@ struct cn_engine {
   int mass_nblk = 64;
   std::vector<void*> allocs;
   hipGraphExec_t step_graph[2] = {nullptr, nullptr};
+  hipGraphExec_t render_graph[2] = {nullptr, nullptr};  // step + render epilogue
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   Rollout ro;               // last rollout plan
   bool ro_fwd_done = false; // a rollout forward has run since the last state change
@@ -214,11 +238,12 @@ cn_status dalloc(cn_engine* e, std::vector<void*>& owner, void** p, size_t bytes
 }
 
 void drop_graphs(cn_engine* e) {
-  for (auto& g : e->step_graph)
-    if (g) {
-      (void)hipGraphExecDestroy(g);
-      g = nullptr;
-    }
+  for (auto* gs : {e->step_graph, e->render_graph})
+    for (int j = 0; j < 2; ++j)
+      if (gs[j]) {
+        (void)hipGraphExecDestroy(gs[j]);
+        gs[j] = nullptr;
+      }
   if (e->ro.graph) (void)hipGraphExecDestroy(e->ro.graph);
   if (e->ro.fwd) (void)hipGraphExecDestroy(e->ro.fwd);
   e->ro = Rollout{};
@@ -263,9 +288,11 @@ StepLaunch class_launch(cn_engine* e, const SizeClass& k, int sin, int sout) {
   return sl;
 }
 
-hipError_t launch_all_steps(cn_engine* e, int sin, int sout) {
+hipError_t launch_all_steps(cn_engine* e, int sin, int sout, bool render = false) {
   for (const SizeClass& k : e->cls) {
-    const hipError_t err = launch_step(k.N, class_launch(e, k, sin, sout), e->stream);
+    StepLaunch sl = class_launch(e, k, sin, sout);
+    sl.render = render;
+    const hipError_t err = launch_step(k.N, sl, e->stream);
     if (err != hipSuccess) return err;
   }
   return hipSuccess;
@@ -298,13 +325,14 @@ cn_status capture(cn_engine* e, hipGraphExec_t* out, F&& fn) {
   return CN_OK;
 }
 
-cn_status enqueue_step(cn_engine* e) {
+cn_status enqueue_step(cn_engine* e, bool render = false) {
   const int cur = e->cur;
-  if (!e->step_graph[cur]) {
-    const cn_status st = capture(e, &e->step_graph[cur], [&] { return launch_all_steps(e, cur, cur ^ 1); });
+  hipGraphExec_t* g = render ? &e->render_graph[cur] : &e->step_graph[cur];
+  if (!*g) {
+    const cn_status st = capture(e, g, [&] { return launch_all_steps(e, cur, cur ^ 1, render); });
     if (st != CN_OK) return st;
   }
-  CN_HIP(hipGraphLaunch(e->step_graph[cur], e->stream));
+  CN_HIP(hipGraphLaunch(*g, e->stream));
   e->cur ^= 1;
   e->steps++;
   e->have_prev = true;
@@ -542,6 +570,8 @@ cn_status create_engine(const cn_engine_desc* descs, uint32_t count, int device,
     d.khalf = p;
     if ((st = dalloc(e, c.allocs, &p, K * N * 8)) != CN_OK) return fail(st);
     d.knyq = p;
+    if ((st = dalloc(e, c.allocs, &p, K * (H + 1) * H * 4)) != CN_OK) return fail(st);
+    d.kreal = (const float*)p;
     if ((st = dalloc(e, c.allocs, &p, K * N * H * 8)) != CN_OK) return fail(st);
     d.uhat = p;
     if ((st = dalloc(e, c.allocs, &p, C * P * 4)) != CN_OK) return fail(st);
@@ -614,12 +644,21 @@ cn_status cn_creature_set_kernel(cn_engine* e, uint32_t i, uint32_t k, const flo
   if (!finite_all(interleaved, 2 * n_complex)) return set_error(CN_ERR_INVALID_ARG, "kernel spectrum not finite");
   std::vector<float> half(N * H * 2), nyq(N * 2);
   project_kernel(interleaved, c.N, half.data(), nyq.data());
+  std::vector<float> real;
+  const bool sym = real_even_spectrum(half.data(), nyq.data(), c.N, real);
   CN_HIP(hipSetDevice(e->device));
   CN_HIP(hipMemcpyAsync((char*)c.dev.khalf + (size_t)k * N * H * 8, half.data(), half.size() * 4,
                         hipMemcpyHostToDevice, e->stream));
   CN_HIP(hipMemcpyAsync((char*)c.dev.knyq + (size_t)k * N * 8, nyq.data(), nyq.size() * 4, hipMemcpyHostToDevice,
                         e->stream));
+  if (sym)
+    CN_HIP(hipMemcpyAsync((char*)c.dev.kreal + (size_t)k * (H + 1) * H * 4, real.data(), real.size() * 4,
+                          hipMemcpyHostToDevice, e->stream));
   CN_HIP(hipStreamSynchronize(e->stream));
+  if (c.dev.ksym[k] != (sym ? 1 : 0)) {
+    c.dev.ksym[k] = sym ? 1 : 0;
+    return upload_desc(e, (int)i);
+  }
   return CN_OK;
 }
 
@@ -699,6 +738,57 @@ cn_status cn_sync(cn_engine* e) {
   CN_HIP(hipSetDevice(e->device));
   CN_HIP(hipStreamSynchronize(e->stream));
   return CN_OK;
+}
+
+// Allocates every creature's packed frame (zeroed, so byte 3 of each pixel stays 0).
+static cn_status ensure_render(cn_engine* e) {
+  for (size_t i = 0; i < e->cr.size(); ++i) {
+    Creature& c = e->cr[i];
+    if (c.dev.render) continue;
+    void* p = nullptr;
+    const cn_status st = dalloc(e, c.allocs, &p, c.plane() * 4);
+    if (st != CN_OK) return st;
+    c.dev.render = (unsigned int*)p;
+    if (upload_desc(e, (int)i) != CN_OK) return CN_ERR_HIP;
+  }
+  return CN_OK;
+}
+
+cn_status cn_iterate_and_render(cn_engine* e) {
+  clear_error();
+  if (!e) return set_error(CN_ERR_INVALID_ARG, "null engine");
+  CN_HIP(hipSetDevice(e->device));
+  const cn_status st = ensure_render(e);
+  if (st != CN_OK) return st;
+  return enqueue_step(e, true);
+}
+
+cn_status cn_creature_download_render(cn_engine* e, uint32_t i, uint8_t* rgb, size_t n) {
+  clear_error();
+  cn_status st = check_creature(e, i);
+  if (st != CN_OK) return st;
+  if (!rgb) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  const Creature& c = e->cr[i];
+  const size_t P = c.plane();
+  if (n != P * 3) return set_error(CN_ERR_INVALID_ARG, "output has %zu bytes, expected %zu", n, P * 3);
+  if (!c.dev.render) {  // no frame rendered yet: the zero-initialised frame
+    std::memset(rgb, 0, n);
+    return CN_OK;
+  }
+  CN_HIP(hipSetDevice(e->device));
+  std::vector<uint32_t> packed(P);
+  CN_HIP(hipMemcpyAsync(packed.data(), c.dev.render, P * 4, hipMemcpyDeviceToHost, e->stream));
+  CN_HIP(hipStreamSynchronize(e->stream));
+  for (size_t j = 0; j < P; ++j) {
+    rgb[3 * j] = (uint8_t)(packed[j] >> 16);
+    rgb[3 * j + 1] = (uint8_t)(packed[j] >> 8);
+    rgb[3 * j + 2] = (uint8_t)packed[j];
+  }
+  return CN_OK;
+}
+
+cn_status cn_download_render(cn_engine* e, uint8_t* rgb, size_t n) {
+  return cn_creature_download_render(e, 0, rgb, n);
 }
 
 // One directly launched step of every creature with events around every kernel of the first
@@ -846,6 +936,8 @@ cn_status cn_engine_get_info(const cn_engine* e, cn_engine_info* out) {
   out->num_kernels = e->cr[0].K;
   out->steps_done = e->steps;
   out->device_bytes = e->bytes;
+  out->compact_kernels = 0;
+  for (uint32_t k = 0; k < e->cr[0].K; ++k) out->compact_kernels += e->cr[0].dev.ksym[k] ? 1u : 0u;
   return CN_OK;
 }
 

@@ -14,6 +14,7 @@ struct StepLaunch {
   const float2* tw = nullptr;          // kTwN-entry twiddle table
   bool write_u = false;                // also write the real convolution results to desc.u
   bool skip_diffusion = false;         // stop after the affinity (conv readback)
+  bool render = false;                 // diffusion also packs channels 0..2 into desc.render
   // Launch geometry (0: built-in default): values per thread of the forward (R2C) and
   // inverse (C2R) row FFTs and of the column FFTs, column tile width, diffusion tile height.
   int fwd_e = 0, inv_e = 0, col_e = 0, col_tc = 0, diff_th = 0;

@@ -75,13 +75,25 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
 
   // Kernel spectra are prefetched one kernel ahead: the loads of kernel k+1's tile are in
   // flight while kernel k's inverse transform runs (the first ones during the forward one).
+  // Kernels with a real spectrum that is even in ky are read from the compact real copy
+  // (a quarter of the bytes); the product is the same.
   const float2* khalf = reinterpret_cast<const float2*>(d.khalf);
-  float2 nk[E];
-  {
-    const float2* kh0 = khalf + (size_t)d.src_k[kb] * N * HALF;
+  auto load_kernel = [&](int k, float2* out) {
+    if (d.ksym[k]) {
+      const float* kr = d.kreal + (size_t)k * (HALF + 1) * HALF;
 #pragma unroll
-    for (int i = 0; i < E; ++i) nk[i] = kh0[(size_t)out_pos<L, E, false>(t, i) * HALF + x];
-  }
+      for (int i = 0; i < E; ++i) {
+        const int y = out_pos<L, E, false>(t, i);
+        out[i] = make_float2(kr[(size_t)(y <= HALF ? y : N - y) * HALF + x], 0.0f);
+      }
+    } else {
+      const float2* kh = khalf + (size_t)k * N * HALF;
+#pragma unroll
+      for (int i = 0; i < E; ++i) out[i] = kh[(size_t)out_pos<L, E, false>(t, i) * HALF + x];
+    }
+  };
+  float2 nk[E];
+  load_kernel(d.src_k[kb], nk);
   __shared__ float2 lds[LdsMap<L, TC, true>::size];
   fft_run<L, E, TC, true, false, kTwN>(v, lds, t, lane, tw);
 
@@ -97,11 +109,7 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
     float2 w[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) w[i] = c_mul(v[i], nk[i]);
-    if (kk + 1 < ke) {
-      const float2* khn = khalf + (size_t)d.src_k[kk + 1] * N * HALF;
-#pragma unroll
-      for (int i = 0; i < E; ++i) nk[i] = khn[(size_t)out_pos<L, E, false>(t, i) * HALF + x];
-    }
+    if (kk + 1 < ke) load_kernel(d.src_k[kk + 1], nk);
     if (edge) {
       __syncthreads();
       if (lane == 0) {
@@ -210,9 +218,31 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
   e = (a - (s - bb)) + (b - bb);
 }
 
+// IEEE round-to-nearest f32 square root of x in [0, 1].  The hardware square root is within
+// 1 ulp; the correctly rounded result t is the candidate in {s - ulp, s, s + ulp} whose rounding
+// interval holds sqrt(x), i.e. lo^2 <= x <= hi^2 for the midpoints lo, hi to its neighbours.
+// Midpoints have 25 significant bits, so their squares and the comparisons are exact in FP64.
+__device__ __forceinline__ float sqrt_rn01(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  if (!(x > 0.0f)) return 0.0f;
+  const float dn = __uint_as_float(__float_as_uint(s) - 1u), up = __uint_as_float(__float_as_uint(s) + 1u);
+  const double xd = (double)x;
+  const double lo = 0.5 * ((double)dn + (double)s), hi = 0.5 * ((double)s + (double)up);
+  if (xd < lo * lo) return dn;
+  if (xd > hi * hi) return up;
+  return s;
+}
+
+// One colour byte: u32(sqrt(clamp(1.5 v, 0, 1)) * 255) with IEEE f32 sqrt and truncation
+// (compute_1024.wgsl:438-440).  NaN maps to 0.
+__device__ __forceinline__ unsigned char render_byte(float v) {
+  const float x = fminf(fmaxf(v * 1.5f, 0.0f), 1.0f);
+  return (unsigned char)(unsigned int)(sqrt_rn01(x) * 255.0f);
+}
+
 template <int TH>
 __global__ void __launch_bounds__(kDiffThreads)
-k_diffusion(const CreatureDev* __restrict__ descs, int sin, int sout, int N) {
+k_diffusion(const CreatureDev* __restrict__ descs, int sin, int sout, int N, int render) {
   constexpr int TW = kDiffTW;
   const CreatureDev& d = descs[blockIdx.z];
   if (d.N != N) return;
@@ -227,6 +257,9 @@ k_diffusion(const CreatureDev* __restrict__ descs, int sin, int sout, int N) {
   const float* s = slot_ptr(d, sin) + c * plane;
   float* o = slot_ptr(d, sout) + c * plane;
 
+  // Render epilogue (render.rs:145-153, compute_1024.wgsl:429-443): channel c < 3 writes its
+  // byte of the packed 0x00RRGGBB pixel (little endian: R is byte 2, B byte 0).
+  unsigned char* rgb = (render && c < 3 && d.render) ? reinterpret_cast<unsigned char*>(d.render) : nullptr;
   __shared__ float sA[TH + 4][TW + 5];
   __shared__ float sQ[TH + 2][TW + 3];
   __shared__ float sR[TH + 2][TW + 3];
@@ -267,7 +300,10 @@ k_diffusion(const CreatureDev* __restrict__ descs, int sin, int sout, int N) {
     const float t = a * sh;
     const float err = fmaf(a, sh, -t);
     const float v = t + (err + fmaf(a, sl, sR[r + 1][cc + 1]));
-    if (gy < N && gx < N) o[(size_t)gy * N + gx] = v;
+    if (gy < N && gx < N) {
+      o[(size_t)gy * N + gx] = v;
+      if (rgb) rgb[((size_t)gy * N + gx) * 4 + (2 - c)] = render_byte(v);
+    }
   }
 }
 
@@ -385,9 +421,9 @@ static hipError_t launch_step_n(const StepLaunch& sl, hipStream_t st) {
     const int tiles_x = (N + kDiffTW - 1) / kDiffTW, tiles_y = (N + th - 1) / th;
     const dim3 g_dif(tiles_x, tiles_y * sl.max_c, sl.count);
     switch (th) {
-      case 8: k_diffusion<8><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N); break;
-      case 16: k_diffusion<16><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N); break;
-      case 32: k_diffusion<32><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N); break;
+      case 8: k_diffusion<8><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N, sl.render); break;
+      case 16: k_diffusion<16><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N, sl.render); break;
+      case 32: k_diffusion<32><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N, sl.render); break;
       default: return hipErrorInvalidValue;
     }
   }

@@ -8,10 +8,14 @@
 //   khalf   [K][N][N/2]  c64   Hermitian part of each kernel spectrum, pre-scaled by 1/N^2,
 //                              columns kx = 0 .. N/2-1
 //   knyq    [K][N]       c64   the kx = N/2 (Nyquist) column of the same spectra
+//   kreal   [K][N/2+1][N/2] f32 compact copy for kernels whose spectrum is real and even in ky
+//                              (rows ky = 0 .. N/2; row N-ky equals row ky); ksym[k] = 1
 //   uhat    [K][N][N/2]  c64   row spectra of every convolution result (after the inverse
 //                              column pass); slot kx = 0 packs (DC, Nyquist) as (re, im)
 //   A       [C][N][N]    f32   affinity exp(temp * sum_k w_k G_k) per destination channel
 //   u       [K][N][N]    f32   convolution results (readback and adjoint only)
+//   render  [N][N]       u32   packed 0x00RRGGBB frame of channels 0..2 (allocated on the first
+//                              iterate_and_render; the diffusion pass writes bytes 2, 1, 0)
 // Adjoint buffers (allocated on first rollout):
 //   grad    [2][C][N][N] f32   ping-pong dL/dstate
 //   gaff    [C][N][N]    f32   dL/d(pre-exp affinity)
@@ -36,9 +40,11 @@ struct CreatureDev {
   void* spec;          // float2*
   const void* khalf;   // float2*
   const void* knyq;    // float2*
+  const float* kreal;
   void* uhat;          // float2*
   float* A;
   float* u;
+  unsigned int* render;
   float* grad;
   float* gaff;
   float* dsd;
@@ -61,6 +67,7 @@ struct CreatureDev {
   float sigma[kMaxKernels];
   float inv_s2[kMaxKernels];   // 1 / sigma^2
   float w[kMaxKernels];
+  int ksym[kMaxKernels];       // 1: kernel k is read from kreal (real, even in ky)
 };
 
 #ifdef __HIPCC__

@@ -139,9 +139,20 @@ class _Engine:
             _ptr(w) if w is not None else None, K, 1 if device else 0))
         return {"state0": state, "mu": mu, "sigma": sigma, "weights": w}
 
+    def creature_download_render(self, i):
+        """RGB24 frame of the last iterate_and_render, (N, N, 3) uint8."""
+        n = self._shapes[i][1]
+        out = np.empty((n, n, 3), dtype=np.uint8)
+        check(lib().cn_creature_download_render(self._h, int(i), out.ctypes.data, out.size))
+        return out
+
     # --- all creatures -------------------------------------------------------------------------
     def iterate(self):
         check(lib().cn_iterate(self._h))
+
+    def iterate_and_render(self):
+        """One step; its diffusion pass also packs channels 0..2 into each creature's frame."""
+        check(lib().cn_iterate_and_render(self._h))
 
     def run_steps(self, n):
         check(lib().cn_run_steps(self._h, int(n)))
@@ -236,6 +247,10 @@ class GpuMaceLenia(_Engine):
         check(lib().cn_download_conv_real(self._h, out.ctypes.data, out.size))
         return out
 
+    def download_render(self):
+        """RGB24 bytes of the last rendered frame, N*N*3 uint8 (mod.rs:302-335)."""
+        return self.creature_download_render(0).reshape(-1)
+
     def mass_fp64(self):
         """Per-channel FP64 compensated mass of the current state."""
         return self.creature_mass(0)
@@ -244,7 +259,7 @@ class GpuMaceLenia(_Engine):
         i = EngineInfo()
         check(lib().cn_engine_get_info(self._h, ctypes.byref(i)))
         return {"n": i.n, "num_channels": i.num_channels, "num_kernels": i.num_kernels,
-                "steps_done": i.steps_done, "device_bytes": i.device_bytes}
+                "steps_done": i.steps_done, "device_bytes": i.device_bytes, "compact_kernels": i.compact_kernels}
 
 
 class CreatureBatch(_Engine):

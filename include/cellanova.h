@@ -92,6 +92,18 @@ cn_status cn_download_channels(cn_engine* e, float* out, size_t n);
  * most recent step started from; zeros before the first step. */
 cn_status cn_download_conv_real(cn_engine* e, float* out, size_t n);
 
+/* GpuMaceLenia::iterate_and_render (mod.rs:390-432): one step whose diffusion pass also packs
+ * channels 0..2 of the new state into a 0x00RRGGBB frame, pixel = (r << 16) | (g << 8) | b with
+ * each byte u32(sqrt(clamp(1.5 v, 0, 1)) * 255) (render.rs:145-153, compute_1024.wgsl:429-443;
+ * IEEE f32 sqrt, truncation; channels >= C read as 0).  Every creature of a batch is rendered.
+ * Asynchronous like cn_iterate. */
+cn_status cn_iterate_and_render(cn_engine* e);
+
+/* GpuMaceLenia::download_render (mod.rs:302-335): the last rendered frame as RGB24 bytes, row
+ * major, n = 3 N^2.  All zeros before the first cn_iterate_and_render.  Synchronises. */
+cn_status cn_download_render(cn_engine* e, uint8_t* rgb, size_t n);
+cn_status cn_creature_download_render(cn_engine* e, uint32_t i, uint8_t* rgb, size_t n);
+
 /* New: per-channel total mass of the current state, FP64 Neumaier-compensated sums on the
  * device (out[c], c < C).  Synchronises. */
 cn_status cn_mass_fp64(cn_engine* e, double* out, size_t n_channels);
@@ -101,6 +113,8 @@ typedef struct cn_engine_info {
   uint32_t n, num_channels, num_kernels;
   uint64_t steps_done;
   uint64_t device_bytes;   /* device memory held by the engine */
+  uint32_t compact_kernels; /* kernels of creature 0 whose spectrum is real and even in ky and
+                               is read from the compact real copy (a quarter of the bytes) */
 } cn_engine_info;
 cn_status cn_engine_get_info(const cn_engine* e, cn_engine_info* out);
 

@@ -8,7 +8,7 @@ Two independent CPU restatements of one DiffusionLenia step:
     train/lenia_org.py:232-275) with numpy.fft in FP64, used where the C++ FP64 oracle is too
     slow (N >= 1024);
 plus brute-force pins that share no code with either: a matrix DFT and a direct circular
-convolution in FP64.
+convolution in FP64; and the render epilogue's colour pack (render_rgb).
 """
 import ctypes
 import os
@@ -205,6 +205,19 @@ def conv_direct(channel, kernel):
         for b in range(n):
             if k[a, b] != 0.0:
                 out += k[a, b] * np.roll(np.roll(ch, a, axis=0), b, axis=1)
+    return out
+
+
+def render_rgb(state):
+    """RGB24 frame of channels 0..2 (render.rs:145-153 dispatch, compute_1024.wgsl:429-443 and the
+    unpacking of mod.rs:325-330): byte = u32(sqrt(clamp(1.5 c, 0, 1)) * 255) in f32, IEEE sqrt,
+    truncation toward zero; channels beyond C read as 0.  Returns (N, N, 3) uint8."""
+    st = np.asarray(state, dtype=np.float32)
+    C, n = st.shape[0], st.shape[1]
+    out = np.zeros((n, n, 3), dtype=np.uint8)
+    for c in range(min(C, 3)):
+        x = np.clip(st[c] * np.float32(1.5), np.float32(0.0), np.float32(1.0))
+        out[..., c] = (np.sqrt(x) * np.float32(255.0)).astype(np.float32).astype(np.uint32).astype(np.uint8)
     return out
 
 

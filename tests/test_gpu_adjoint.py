@@ -67,6 +67,24 @@ def test_rollout_grad_matches_torch_fp64(cn, n, C, steps, ckpt):
         assert errs[k] <= 1e-4, (k, errs[k])
 
 
+def test_rollout_grad_with_shifted_kernels(cn):
+    """Kernels whose spectra are not real and even (translated kernels) take the full complex
+    path in the adjoint column pass (product with conj(K^)); checked against torch FP64."""
+    from test_gpu_step import shift_kernels
+    cr, tg = _pair(cn, 64, 3, R=10, seed=21, patch=32)
+    shift_kernels(cr, [0, 4, 8], dx=2, dy=5)
+    tgt = tg["state"]
+    loss_t, gs_t, gm_t, gsg_t, gw_t = _torch_grads(cr, tgt, 4)
+    loss, g, _ = _engine_grads(cn, cr, tgt, 4, 2)
+    errs = {"loss": abs(loss - loss_t) / loss_t,
+            "state0": _rel(g["state0"].reshape(gs_t.shape), gs_t),
+            "mu": _rel(g["mu"], gm_t), "sigma": _rel(g["sigma"], gsg_t), "weights": _rel(g["weights"], gw_t)}
+    print("shifted kernels: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+    assert errs["loss"] <= 1e-5
+    for k in ("state0", "mu", "sigma", "weights"):
+        assert errs[k] <= 1e-4, (k, errs[k])
+
+
 def test_checkpoint_stride_does_not_change_gradients(cn):
     cr, tg = _pair(cn, 64, 3, R=10, seed=5, patch=32)
     ref = None

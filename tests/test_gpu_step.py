@@ -65,6 +65,50 @@ def test_step_parity_c5k25(cn, orc):
     assert max(errs) <= TOL
 
 
+def shift_kernels(cr, ks, dx=3, dy=-2):
+    """Translate kernels `ks` by (dx, dy) cells: their spectra gain a phase, so they are no
+    longer real and even and take the full complex spectrum path; the rest stay on the compact
+    real copy."""
+    n = cr["state"].shape[1]
+    j = np.arange(n)
+    ph = np.exp(-2j * np.pi * (j[:, None] * dy + j[None, :] * dx) / n)
+    kf = np.array(cr["kernels_fft"], dtype=np.complex128)
+    for k in ks:
+        kf[k] = (kf[k].reshape(n, n) * ph).reshape(kf[k].shape)
+    cr["kernels_fft"] = kf.astype(np.complex64)
+    return cr
+
+
+def _real_even(spec):
+    n = int(round(np.sqrt(spec.size)))
+    spec = spec.reshape(n, n)
+    h = 0.5 * (spec + np.conj(spec[np.r_[0, len(spec) - 1:0:-1]][:, np.r_[0, len(spec) - 1:0:-1]]))
+    tol = 2.0 ** -23 * np.abs(h).max()
+    return np.abs(h.imag).max() <= tol and np.abs(h.real - h.real[np.r_[0, len(h) - 1:0:-1]]).max() <= tol
+
+
+@pytest.mark.parametrize("n", [64, 256])
+def test_compact_and_complex_kernel_paths(cn, orc, n):
+    cr = _creature(cn, n, "c3k9", max(4, n // 6), seed=7)
+    assert all(_real_even(np.asarray(cr["kernels_fft"][k], np.complex128)) for k in range(9))
+    shift_kernels(cr, [1, 5, 6])
+    assert not _real_even(np.asarray(cr["kernels_fft"][1], np.complex128))
+    errs = _teacher_forced(cn, orc, cr, steps=3)
+    print(f"n={n} mixed compact/complex kernels: rel L-inf per step {['%.2e' % e for e in errs]}")
+    assert max(errs) <= TOL
+    eng = cn.make_engine(cr)
+    assert eng.info()["compact_kernels"] == 6
+    eng.iterate()
+    conv = eng.download_conv_buffer().reshape(9, n, n) / (n * n)
+    want = orc.conv_f64(cr["state"].astype(np.float64), cr["kernels_fft"], cr["src"])
+    assert orc.rel_linf(conv, want) <= 1e-6
+    shift_kernels(cr, [1, 5, 6], dx=-3, dy=2)          # back to centred: all compact again
+    for k in (1, 5, 6):
+        eng.set_kernel(cr["kernels_fft"][k], k)
+    assert eng.info()["compact_kernels"] == 9
+    eng.close()
+
+
 def test_conv_buffer_matches_oracle(cn, orc):
     cr = cn.baseline_creature(64, num_channels=3, radius=9, seed=5, patch=32)
     eng = cn.make_engine(cr)

@@ -217,3 +217,36 @@ def test_golden_fixture_planted_wrong_mapping_is_caught(orc):
     s1 = orc.run_f64(z["state"].astype(np.float64), z["kernels_fft"], z["dst"], z["src"], z["mu"], z["sigma"],
                      z["weights"], temp=float(z["temp"]))
     assert orc.rel_linf(s1, z["state1"]) > 1e-3
+
+
+# --- render epilogue colour pack (render.rs:145-153, compute_1024.wgsl:429-443) ---------------
+def test_render_rgb_closed_forms(orc):
+    st = np.zeros((3, 2, 2), np.float32)
+    st[:, 0, 0] = (0.5, 0.0, 2.0 / 3.0)     # floor(255 sqrt(0.75)) = 220; 1.5 * 2/3 clamps to 1 -> 255
+    st[:, 0, 1] = (-1.0, 7.0, np.nan)       # clamp below and above; NaN gives 0 here
+    st[:, 1, 0] = (1.0 / 6.0, 0.01, 0.1)    # 255 sqrt(0.25) = 127.5 truncates to 127
+    rgb = orc.render_rgb(st)
+    assert rgb[0, 0].tolist() == [220, 0, 255]
+    assert rgb[0, 1].tolist() == [0, 255, 0]
+    assert rgb[1, 0].tolist() == [127, int(np.floor(255 * np.sqrt(0.015))), int(np.floor(255 * np.sqrt(0.15)))]
+    assert rgb[1, 1].tolist() == [0, 0, 0]
+
+
+def test_render_rgb_matches_fp64_formula_away_from_integer_boundaries(orc):
+    c = np.linspace(-0.05, 0.7, 3 * 40000, dtype=np.float32).reshape(3, 200, 200)
+    got = orc.render_rgb(c).transpose(2, 0, 1).astype(np.int64)
+    v = 255.0 * np.sqrt(np.clip(1.5 * c.astype(np.float64), 0.0, 1.0))
+    want = np.floor(v).astype(np.int64)
+    clamped = (c <= 0) | (1.5 * c.astype(np.float64) >= 1.0)
+    away = (np.abs(v - np.round(v)) > 1e-4) | clamped   # f32 rounding only matters next to an integer
+    assert away.mean() > 0.99 and clamped.mean() < 0.2
+    assert np.array_equal(got[away], want[away])
+    assert np.abs(got - want).max() <= 1
+
+
+def test_render_rgb_fewer_channels_and_order(orc):
+    st = np.full((1, 4, 4), 0.5, np.float32)
+    rgb = orc.render_rgb(st)
+    assert (rgb[..., 0] == 220).all() and (rgb[..., 1:] == 0).all()
+    st5 = np.stack([np.full((4, 4), v, np.float32) for v in (0.0, 0.5, 2.0 / 3.0, 1.0, 1.0)])
+    assert orc.render_rgb(st5)[0, 0].tolist() == [0, 220, 255]   # channels 3.. are not drawn
