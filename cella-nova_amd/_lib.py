@@ -86,6 +86,10 @@ SIGNATURES = {
     "cn_load_config": (c_int, [c_char_p, POINTER(POINTER(Config))]),
     "cn_free_config": (None, [POINTER(Config)]),
     "cn_load_kernels": (c_int, [c_char_p, c_uint32, c_uint32, c_void_p, c_size_t]),
+    "cn_write_kernels": (c_int, [c_char_p, c_void_p, c_uint32, c_uint32]),
+    "cn_write_config": (c_int, [c_char_p, POINTER(Config)]),
+    "cn_creature_generate": (c_int, [c_void_p, c_uint32, POINTER(CreatureParams)]),
+    "cn_creature_download_kernels": (c_int, [c_void_p, c_uint32, c_void_p, c_size_t]),
     "cn_generate_creature": (c_int, [POINTER(CreatureParams), c_void_p, c_void_p, c_void_p, c_void_p,
                                      c_void_p, c_void_p]),
     "cn_debug_fail_next_hip": (None, [c_int]),

@@ -1,5 +1,6 @@
 """Loader mirror of ml-rs/config.rs: SimConfig, load_config (with the c0/c1 swap, config.rs:40)
-and load_kernels (with spectral zero-padding, config.rs:56-125), implemented in the C ABI."""
+and load_kernels (with spectral zero-padding, config.rs:56-125), and the writers of the same two
+files (train/generate_kernel_json.py:171-213), implemented in the C ABI."""
 import ctypes
 from dataclasses import dataclass
 
@@ -48,3 +49,30 @@ def load_kernels(path, num_kernels, target_size):
     out = np.empty(2 * num_kernels * target_size * target_size, dtype=np.float32)
     check(lib().cn_load_kernels(str(path).encode(), int(num_kernels), int(target_size), out.ctypes.data, out.size))
     return out.view(np.complex64).reshape(num_kernels, target_size * target_size)
+
+
+def write_kernels(path, spectra):
+    """K spectra (K, n*n) complex64 -> .bin of little-endian (re, im) f32 pairs (config.rs:49-55)."""
+    a = np.ascontiguousarray(np.asarray(spectra, dtype=np.complex64))
+    K = a.shape[0]
+    n = int(round(np.sqrt(a[0].size)))
+    if n * n != a[0].size:
+        raise ValueError("each spectrum must hold n*n values")
+    f = a.reshape(-1).view(np.float32)
+    check(lib().cn_write_kernels(str(path).encode(), f.ctypes.data, K, n))
+
+
+def write_config(path, cfg):
+    """SimConfig (loader convention: c0 = input channel) -> creature JSON that load_config reads
+    back to the same values."""
+    C, K = int(cfg.num_channels), int(cfg.num_kernels)
+    seeds = np.ascontiguousarray(np.asarray(cfg.seed_channels, dtype=np.float64).reshape(C, -1))
+    c0, c1 = (np.ascontiguousarray(np.asarray(a, np.uint32)) for a in (cfg.c0, cfg.c1))
+    mu, sg, w = (np.ascontiguousarray(np.asarray(a, np.float32))
+                 for a in (cfg.growth_mu, cfg.growth_sigma, cfg.growth_weights))
+    P = ctypes.POINTER
+    c = Config(int(cfg.seed_size), C, K, seeds.ctypes.data_as(P(ctypes.c_double)), seeds.shape[1],
+               c0.ctypes.data_as(P(ctypes.c_uint32)), c1.ctypes.data_as(P(ctypes.c_uint32)),
+               mu.ctypes.data_as(P(ctypes.c_float)), sg.ctypes.data_as(P(ctypes.c_float)),
+               w.ctypes.data_as(P(ctypes.c_float)))
+    check(lib().cn_write_config(str(path).encode(), ctypes.byref(c)))

@@ -16,8 +16,8 @@ def channel_maps(num_channels, num_kernels=None):
     return k % num_channels, (k // num_channels) % num_channels
 
 
-def generate_creature(n, num_channels, num_kernels, src, dst, radius, patch=128, seed=0, creature=0,
-                      spectra=True, spatial=False, state=True):
+def creature_params(n, num_channels, num_kernels, src, dst, radius, patch=128, seed=0, creature=0):
+    """cn_creature_params for the C ABI; returns (params, keep-alive arrays)."""
     src = np.ascontiguousarray(src, dtype=np.uint32)
     dst = np.ascontiguousarray(dst, dtype=np.uint32)
     rad = np.ascontiguousarray(np.broadcast_to(np.asarray(radius, dtype=np.uint32), (num_kernels,)))
@@ -26,6 +26,12 @@ def generate_creature(n, num_channels, num_kernels, src, dst, radius, patch=128,
                        dst.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
                        rad.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
                        int(patch), int(seed), int(creature))
+    return p, (src, dst, rad)
+
+
+def generate_creature(n, num_channels, num_kernels, src, dst, radius, patch=128, seed=0, creature=0,
+                      spectra=True, spatial=False, state=True):
+    p, (src, dst, rad) = creature_params(n, num_channels, num_kernels, src, dst, radius, patch, seed, creature)
     P = n * n
     kf = np.empty(2 * num_kernels * P, np.float32) if spectra else None
     sp = np.empty(num_kernels * P, np.float64) if spatial else None
@@ -67,3 +73,16 @@ def make_engine(cr, temp=1.0, device=0):
     for c in range(cr["num_channels"]):
         e.upload_channel(cr["state"][c], c)
     return e
+
+
+def creature_spec(n, num_channels, radius, patch=128, seed=0, creature=0, src=None, dst=None):
+    """Shape and generator keys of a creature (no data): K = C^2 with the reference maps unless
+    src/dst are given.  CreatureBatch.generated builds and fills creatures from these on the
+    device."""
+    if src is None:
+        src, dst = channel_maps(num_channels)
+    K = len(src)
+    return {"n": int(n), "num_channels": int(num_channels), "num_kernels": K,
+            "src": np.asarray(src, np.uint32), "dst": np.asarray(dst, np.uint32),
+            "radius": np.broadcast_to(np.asarray(radius, np.uint32), (K,)).copy(), "patch": int(patch),
+            "seed": int(seed), "creature": int(creature)}

@@ -32,4 +32,9 @@ inline bool finite_all(const float* p, size_t n) {
   return true;
 }
 
+// Synthetic creatures (creature.cpp): parameter checks and growth parameters of the
+// benchmark distribution, shared by the host and device generators.
+cn_status check_creature_params(const cn_creature_params* p);
+void generate_growth(const cn_creature_params* p, float* mu, float* sigma, float* weights);
+
 }  // namespace cn

@@ -9,22 +9,10 @@
 #include <thread>
 #include <vector>
 #include "common.h"
+#include "prng.h"
 
 namespace cn {
 namespace {
-
-// Counter-based generator: splitmix64 finaliser chained over (seed, creature, stream, index).
-inline uint64_t mix64(uint64_t z) {
-  z += 0x9E3779B97F4A7C15ull;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-inline double uniform01(uint64_t seed, uint64_t creature, uint64_t stream, uint64_t index) {
-  const uint64_t h = mix64(mix64(mix64(mix64(seed) + creature) + stream) + index);
-  return (double)(h >> 11) * (1.0 / 9007199254740992.0);  // [0, 1)
-}
-enum Stream : uint64_t { kRings = 1, kBeta = 2, kMu = 3, kSigma = 4, kWeight = 5, kState = 6 };
 
 // In-place radix-2 complex FFT of length n (power of two), FP64, forward (exp(-i...)).
 void fft1d(double* re, double* im, size_t n, const std::vector<double>& cs, const std::vector<double>& sn,
@@ -76,26 +64,12 @@ void fft2d(std::vector<double>& re, std::vector<double>& im, size_t n) {
 }
 
 }  // namespace
-}  // namespace cn
 
-using namespace cn;
-
-extern "C" cn_status cn_generate_creature(const cn_creature_params* p, float* kernels_fft, double* spatial,
-                                          float* state, float* mu, float* sigma, float* weights) {
-  clear_error();
-  if (!p) return set_error(CN_ERR_INVALID_ARG, "null params");
-  const uint32_t n = p->n, C = p->num_channels, K = p->num_kernels;
-  if (!is_pow2(n) || n < 2 || n > 8192) return set_error(CN_ERR_UNSUPPORTED, "n = %u must be a power of two", n);
-  if (C < 1 || C > 16 || K < 1 || K > 256) return set_error(CN_ERR_UNSUPPORTED, "C = %u, K = %u out of range", C, K);
-  if (!p->src || !p->dst || !p->radius) return set_error(CN_ERR_INVALID_ARG, "null src/dst/radius");
-  for (uint32_t k = 0; k < K; ++k) {
-    if (p->src[k] >= C || p->dst[k] >= C) return set_error(CN_ERR_INVALID_ARG, "src/dst[%u] out of range", k);
-    if (p->radius[k] < 1) return set_error(CN_ERR_INVALID_ARG, "radius[%u] must be >= 1", k);
-  }
-  const size_t N = n, P = N * N;
+// Growth parameters mu ~ U[0.05, 0.15], sigma ~ U[0.03, 0.06] and Dirichlet(1) weights over
+// the kernels that share a destination channel (normalised i.i.d. Exp(1) draws).
+void generate_growth(const cn_creature_params* p, float* mu, float* sigma, float* weights) {
+  const uint32_t K = p->num_kernels;
   const uint64_t sd = p->seed, cr = p->creature;
-
-  // Growth parameters and Dirichlet(1) weights per destination channel.
   std::vector<double> ew(K);
   for (uint32_t k = 0; k < K; ++k) ew[k] = -std::log1p(-uniform01(sd, cr, kWeight, k));  // Exp(1)
   for (uint32_t k = 0; k < K; ++k) {
@@ -106,37 +80,56 @@ extern "C" cn_status cn_generate_creature(const cn_creature_params* p, float* ke
     if (sigma) sigma[k] = (float)(0.03 + 0.03 * uniform01(sd, cr, kSigma, k));
     if (weights) weights[k] = (float)(ew[k] / tot);
   }
+}
+
+cn_status check_creature_params(const cn_creature_params* p) {
+  if (!p) return set_error(CN_ERR_INVALID_ARG, "null params");
+  const uint32_t n = p->n, C = p->num_channels, K = p->num_kernels;
+  if (!is_pow2(n) || n < 2 || n > 8192) return set_error(CN_ERR_UNSUPPORTED, "n = %u must be a power of two", n);
+  if (C < 1 || C > 16 || K < 1 || K > 256) return set_error(CN_ERR_UNSUPPORTED, "C = %u, K = %u out of range", C, K);
+  if (!p->src || !p->dst || !p->radius) return set_error(CN_ERR_INVALID_ARG, "null src/dst/radius");
+  for (uint32_t k = 0; k < K; ++k) {
+    if (p->src[k] >= C || p->dst[k] >= C) return set_error(CN_ERR_INVALID_ARG, "src/dst[%u] out of range", k);
+    if (p->radius[k] < 1) return set_error(CN_ERR_INVALID_ARG, "radius[%u] must be >= 1", k);
+  }
+  return CN_OK;
+}
+
+}  // namespace cn
+
+using namespace cn;
+
+extern "C" cn_status cn_generate_creature(const cn_creature_params* p, float* kernels_fft, double* spatial,
+                                          float* state, float* mu, float* sigma, float* weights) {
+  clear_error();
+  const cn_status st = check_creature_params(p);
+  if (st != CN_OK) return st;
+  const uint32_t n = p->n, C = p->num_channels, K = p->num_kernels;
+  const size_t N = n, P = N * N;
+  const uint64_t sd = p->seed, cr = p->creature;
+
+  generate_growth(p, mu, sigma, weights);
 
   // Seed: U[0,1) on the central patch of every channel.
   if (state) {
-    const size_t ps = std::min<size_t>(p->patch, N), o = N / 2 - ps / 2;
+    const SeedPatch sp = seed_patch_of(n, p->patch);
     std::memset(state, 0, (size_t)C * P * sizeof(float));
     for (uint32_t c = 0; c < C; ++c)
-      for (size_t i = 0; i < ps; ++i)
-        for (size_t j = 0; j < ps; ++j)
-          state[c * P + (o + i) * N + (o + j)] = (float)uniform01(sd, cr, kState, ((uint64_t)c * ps + i) * ps + j);
+      for (size_t i = 0; i < sp.side; ++i)
+        for (size_t j = 0; j < sp.side; ++j)
+          state[c * P + (sp.origin + i) * N + (sp.origin + j)] = seed_value(sd, cr, c, (uint32_t)i, (uint32_t)j, sp.side);
   }
 
   if (!kernels_fft && !spatial) return CN_OK;
   // Kernels: one worker thread per kernel (bounded by the host's hardware threads).
   auto build = [&](uint32_t k) {
-    const double R = (double)p->radius[k];
-    const int B = 1 + (int)(3.0 * uniform01(sd, cr, kRings, k));  // U{1,2,3}
-    double beta[3];
-    for (int i = 0; i < 3; ++i) beta[i] = 0.3 + 0.7 * uniform01(sd, cr, kBeta, (uint64_t)k * 4 + i);
+    const RingParams rp = ring_params(sd, cr, k, p->radius[k]);
     std::vector<double> re(P, 0.0), im(P, 0.0);
     double tot = 0.0;
     for (size_t y = 0; y < N; ++y) {
-      const double dy = (y < N / 2) ? (double)y : (double)y - (double)N;
+      const double dy = wrap_offset((uint32_t)y, n);
       for (size_t x = 0; x < N; ++x) {
-        const double dx = (x < N / 2) ? (double)x : (double)x - (double)N;
-        const double r = std::sqrt(dx * dx + dy * dy) / R;
-        if (!(r < 1.0)) continue;
-        const double s = B * r;
-        const int ri = (int)std::floor(s);
-        const double f = s - ri;
-        if (!(f > 0.0 && f < 1.0)) continue;
-        const double v = beta[ri] * std::exp(4.0 - 1.0 / (f * (1.0 - f)));
+        const double v = ring_value(rp, wrap_offset((uint32_t)x, n), dy);
         re[y * N + x] = v;
         tot += v;
       }

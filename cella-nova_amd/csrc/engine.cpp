@@ -221,6 +221,7 @@ struct cn_engine {
   std::vector<void*> allocs;
   hipGraphExec_t step_graph[2] = {nullptr, nullptr};
   hipGraphExec_t render_graph[2] = {nullptr, nullptr};  // step + render epilogue
+  RingParams* gen_rings = nullptr;  // device generator parameters, kMaxKernels entries
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   Rollout ro;               // last rollout plan
   bool ro_fwd_done = false; // a rollout forward has run since the last state change
@@ -712,6 +713,88 @@ cn_status cn_creature_set_growth(cn_engine* e, uint32_t i, const float* mu, cons
   fill_growth(c.dev, c.K, mu, sigma, w);
   CN_HIP(hipSetDevice(e->device));
   return upload_desc(e, (int)i);
+}
+
+// ---------------------------------------------------------------------------------------
+// Device creature generator and spectrum readback
+// ---------------------------------------------------------------------------------------
+cn_status cn_creature_generate(cn_engine* e, uint32_t i, const cn_creature_params* p) {
+  clear_error();
+  cn_status st = check_creature(e, i);
+  if (st != CN_OK) return st;
+  if ((st = check_creature_params(p)) != CN_OK) return st;
+  Creature& c = e->cr[i];
+  if (p->n != c.N || p->num_channels != c.C || p->num_kernels != c.K)
+    return set_error(CN_ERR_INVALID_ARG, "params describe %u^2 C%u K%u, creature %u is %u^2 C%u K%u", p->n,
+                     p->num_channels, p->num_kernels, i, c.N, c.C, c.K);
+  for (uint32_t k = 0; k < c.K; ++k)
+    if ((int)p->src[k] != c.dev.src_of[k] || (int)p->dst[k] != c.dev.dst_of[k])
+      return set_error(CN_ERR_INVALID_ARG, "kernel %u: src/dst differ from the creature's", k);
+  std::vector<float> mu(c.K), sg(c.K), w(c.K);
+  generate_growth(p, mu.data(), sg.data(), w.data());
+  if ((st = validate_growth(c.K, mu.data(), sg.data(), w.data())) != CN_OK) return st;
+  std::vector<RingParams> rings(c.K);
+  for (uint32_t k = 0; k < c.K; ++k) rings[k] = ring_params(p->seed, p->creature, k, p->radius[k]);
+  CN_HIP(hipSetDevice(e->device));
+  if (!e->gen_rings) {
+    void* q = nullptr;
+    if ((st = dalloc(e, e->allocs, &q, kMaxKernels * sizeof(RingParams))) != CN_OK) return st;
+    e->gen_rings = (RingParams*)q;
+  }
+  fill_growth(c.dev, c.K, mu.data(), sg.data(), w.data());
+  // A periodic kernel that depends on dx^2 + dy^2 only is even in x and y, so its spectrum is
+  // real and even: every generated kernel uses the compact real copy.
+  for (uint32_t k = 0; k < c.K; ++k) c.dev.ksym[k] = 1;
+  if ((st = upload_desc(e, (int)i)) != CN_OK) return st;
+  const CreatureDev* d = e->cls[c.cls].d_descs + c.idx;
+  CN_HIP(hipMemcpyAsync(e->gen_rings, rings.data(), c.K * sizeof(RingParams), hipMemcpyHostToDevice, e->stream));
+  CN_HIP(launch_generate((int)c.N, d, e->gen_rings, (int)c.K, e->tw, e->stream));
+  const SeedPatch sp = seed_patch_of(c.N, p->patch);
+  CN_HIP(launch_seed(d, e->cur, c.slot_elems(), sp.side, sp.origin, p->seed, p->creature, e->stream));
+  CN_HIP(hipStreamSynchronize(e->stream));
+  e->have_prev = false;
+  e->ro_fwd_done = false;
+  return CN_OK;
+}
+
+cn_status cn_creature_download_kernels(cn_engine* e, uint32_t i, float* out, size_t n_floats) {
+  clear_error();
+  cn_status st = check_creature(e, i);
+  if (st != CN_OK) return st;
+  if (!out) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  const Creature& c = e->cr[i];
+  const size_t N = c.N, H = N / 2, P = N * N;
+  if (n_floats != 2 * c.K * P)
+    return set_error(CN_ERR_INVALID_ARG, "output has %zu floats, expected %zu", n_floats, 2 * c.K * P);
+  CN_HIP(hipSetDevice(e->device));
+  std::vector<float> half(2 * c.K * N * H), nyq(2 * c.K * N);
+  CN_HIP(hipMemcpyAsync(half.data(), c.dev.khalf, half.size() * 4, hipMemcpyDeviceToHost, e->stream));
+  CN_HIP(hipMemcpyAsync(nyq.data(), c.dev.knyq, nyq.size() * 4, hipMemcpyDeviceToHost, e->stream));
+  CN_HIP(hipStreamSynchronize(e->stream));
+  const float sc = (float)P;  // undo the 1/N^2 fold (exact: power of two)
+  for (size_t k = 0; k < c.K; ++k) {
+    const float* hk = half.data() + 2 * k * N * H;
+    const float* nk = nyq.data() + 2 * k * N;
+    float* o = out + 2 * k * P;
+    for (size_t y = 0; y < N; ++y)
+      for (size_t x = 0; x < N; ++x) {
+        float re, im;
+        if (x < H) {
+          re = hk[2 * (y * H + x)];
+          im = hk[2 * (y * H + x) + 1];
+        } else if (x == H) {
+          re = nk[2 * y];
+          im = nk[2 * y + 1];
+        } else {  // Hermitian: K(y, x) = conj K(-y, -x)
+          const size_t ym = (N - y) & (N - 1), xm = N - x;
+          re = hk[2 * (ym * H + xm)];
+          im = -hk[2 * (ym * H + xm) + 1];
+        }
+        o[2 * (y * N + x)] = re * sc;
+        o[2 * (y * N + x) + 1] = im * sc;
+      }
+  }
+  return CN_OK;
 }
 
 cn_status cn_iterate(cn_engine* e) {

@@ -1,6 +1,7 @@
 // Host-side launch interface of the device kernels (shared by the engine and the batch path).
 #pragma once
 #include <hip/hip_runtime.h>
+#include "prng.h"
 #include "types.h"
 
 namespace cn {
@@ -32,6 +33,14 @@ hipError_t launch_mse_grad(const CreatureDev* descs, int count, int sfin, int go
 hipError_t launch_zero_gparam(const CreatureDev* descs, int count, hipStream_t st);
 int row_blocks(int N);  // row-transform blocks per plane (parameter-gradient partial count)
 constexpr int kLossPartials = 256;
+
+// Device creature generator: ring kernels of `desc` (K = desc->K, parameters in `rings`, whose
+// `tot` the launch fills) -> khalf, knyq, kreal; uses desc->uhat as scratch.
+hipError_t launch_generate(int N, const CreatureDev* desc, RingParams* rings, int K, const float2* tw,
+                           hipStream_t st);
+// Seed patch of every channel into state slot `slot` (`cells` = C N^2).
+hipError_t launch_seed(const CreatureDev* desc, int slot, size_t cells, uint32_t side, uint32_t origin,
+                       uint64_t seed, uint64_t creature, hipStream_t st);
 
 hipError_t launch_mass(const CreatureDev* descs, int count, int slot, int max_c, double2* partial, int nblk,
                        hipStream_t st);

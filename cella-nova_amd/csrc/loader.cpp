@@ -375,4 +375,85 @@ cn_status cn_load_kernels(const char* path, uint32_t num_kernels, uint32_t targe
   return CN_OK;
 }
 
+cn_status cn_write_kernels(const char* path, const float* kernels_fft, uint32_t num_kernels, uint32_t n) {
+  clear_error();
+  if (!path || !kernels_fft) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  if (num_kernels == 0 || n == 0) return set_error(CN_ERR_INVALID_ARG, "num_kernels and n must be > 0");
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return set_error(CN_ERR_IO, "cannot open '%s' for writing: %s", path, std::strerror(errno));
+  const size_t count = 2 * (size_t)num_kernels * n * n;  // little-endian f32 on every supported host
+  const size_t wrote = std::fwrite(kernels_fft, sizeof(float), count, f);
+  const bool closed = std::fclose(f) == 0;
+  if (wrote != count || !closed) return set_error(CN_ERR_IO, "write to '%s' failed", path);
+  return CN_OK;
+}
+
+cn_status cn_write_config(const char* path, const cn_config* cfg) {
+  clear_error();
+  if (!path || !cfg) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  const size_t C = cfg->num_channels, K = cfg->num_kernels, P = (size_t)cfg->seed_size * cfg->seed_size;
+  if (cfg->seed_len != P) return set_error(CN_ERR_INVALID_ARG, "seed_len %zu != seed_size^2 = %zu", cfg->seed_len, P);
+  if ((C && P && !cfg->seed_channels) || (K && (!cfg->c0 || !cfg->c1 || !cfg->growth_mu || !cfg->growth_sigma ||
+                                               !cfg->growth_weights)))
+    return set_error(CN_ERR_INVALID_ARG, "null array in config");
+  for (size_t j = 0; j < C * P; ++j)
+    if (!std::isfinite(cfg->seed_channels[j])) return set_error(CN_ERR_INVALID_ARG, "seed value %zu not finite", j);
+  std::string o;
+  o.reserve(C * P * 24 + K * 64 + 256);
+  char num[40];
+  auto put_u = [&](const uint32_t* a) {
+    o += '[';
+    for (size_t k = 0; k < K; ++k) {
+      std::snprintf(num, sizeof num, k ? ", %u" : "%u", a[k]);
+      o += num;
+    }
+    o += ']';
+  };
+  auto put_f = [&](const float* a) {  // 9 significant digits: reads back to the same f32
+    o += '[';
+    for (size_t k = 0; k < K; ++k) {
+      std::snprintf(num, sizeof num, k ? ", %.9g" : "%.9g", (double)a[k]);
+      o += num;
+    }
+    o += ']';
+  };
+  std::snprintf(num, sizeof num, "%u", cfg->seed_size);
+  o += "{\n  \"seed_size\": ";
+  o += num;
+  std::snprintf(num, sizeof num, "%u", cfg->num_channels);
+  o += ",\n  \"num_channels\": ";
+  o += num;
+  std::snprintf(num, sizeof num, "%u", cfg->num_kernels);
+  o += ",\n  \"num_kernels\": ";
+  o += num;
+  o += ",\n  \"seed_channels\": [";
+  for (size_t c = 0; c < C; ++c) {
+    o += c ? ",\n    [" : "\n    [";
+    for (size_t j = 0; j < P; ++j) {
+      std::snprintf(num, sizeof num, j ? ", %.17g" : "%.17g", cfg->seed_channels[c * P + j]);
+      o += num;
+    }
+    o += ']';
+  }
+  o += C ? "\n  ]" : "]";
+  // The loader swaps c0 and c1 (config.rs:40); write the JSON convention so it reads back cfg.
+  o += ",\n  \"c0\": ";
+  put_u(cfg->c1);
+  o += ",\n  \"c1\": ";
+  put_u(cfg->c0);
+  o += ",\n  \"growth_mu\": ";
+  put_f(cfg->growth_mu);
+  o += ",\n  \"growth_sigma\": ";
+  put_f(cfg->growth_sigma);
+  o += ",\n  \"growth_weights\": ";
+  put_f(cfg->growth_weights);
+  o += "\n}\n";
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return set_error(CN_ERR_IO, "cannot open '%s' for writing: %s", path, std::strerror(errno));
+  const size_t wrote = std::fwrite(o.data(), 1, o.size(), f);
+  const bool closed = std::fclose(f) == 0;
+  if (wrote != o.size() || !closed) return set_error(CN_ERR_IO, "write to '%s' failed", path);
+  return CN_OK;
+}
+
 }  // extern "C"

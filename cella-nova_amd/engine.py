@@ -43,6 +43,8 @@ class _Engine:
 
     def _init(self, descs, device):
         self._shapes = [(int(d.num_channels), int(d.n), int(d.num_kernels)) for d in descs]
+        self._maps = [(np.ctypeslib.as_array(d.src, shape=(d.num_kernels,)).copy(),
+                       np.ctypeslib.as_array(d.dst, shape=(d.num_kernels,)).copy()) for d in descs]
         h = ctypes.c_void_p()
         if len(descs) == 1:
             check(lib().cn_engine_create(ctypes.byref(descs[0]), ctypes.byref(h)))
@@ -138,6 +140,24 @@ class _Engine:
             _ptr(mu) if mu is not None else None, _ptr(sigma) if sigma is not None else None,
             _ptr(w) if w is not None else None, K, 1 if device else 0))
         return {"state0": state, "mu": mu, "sigma": sigma, "weights": w}
+
+    def creature_generate(self, i, radius, patch=128, seed=0, creature=0, src=None, dst=None):
+        """Generate creature i in place on the device (kernel spectra, growth parameters, seed
+        state) from the synthetic distribution; src/dst default to the creature's own maps."""
+        from .creatures import creature_params
+        C, n, K = self._shapes[i]
+        if src is None:
+            src, dst = self._maps[i]
+        p, keep = creature_params(n, C, K, src, dst, radius, patch, seed, creature)
+        check(lib().cn_creature_generate(self._h, int(i), ctypes.byref(p)))
+        del keep
+
+    def creature_download_kernels(self, i):
+        """The spectra creature i steps with: (K, N*N) complex64, unscaled (set_kernel's format)."""
+        C, n, K = self._shapes[i]
+        out = np.empty(2 * K * n * n, dtype=np.float32)
+        check(lib().cn_creature_download_kernels(self._h, int(i), out.ctypes.data, out.size))
+        return out.view(np.complex64).reshape(K, n * n)
 
     def creature_download_render(self, i):
         """RGB24 frame of the last iterate_and_render, (N, N, 3) uint8."""
@@ -272,6 +292,18 @@ class CreatureBatch(_Engine):
                        c["weights"], c.get("temp", 1.0), device, self._keep) for c in creatures]
         self._init(descs, device)
         self.count = len(creatures)
+
+    @classmethod
+    def generated(cls, specs, device=0):
+        """A batch whose creatures (creatures.creature_spec dicts) are generated on the device:
+        no host FFTs and no spectrum uploads."""
+        placeholder = [{**sp, "mu": np.full(sp["num_kernels"], 0.1, np.float32),
+                        "sigma": np.full(sp["num_kernels"], 0.05, np.float32),
+                        "weights": np.full(sp["num_kernels"], 1.0, np.float32)} for sp in specs]
+        b = cls(placeholder, device)
+        for i, sp in enumerate(specs):
+            b.creature_generate(i, sp["radius"], sp.get("patch", 128), sp.get("seed", 0), sp.get("creature", 0))
+        return b
 
     def load(self, i, cr):
         """Upload a generated creature's kernel spectra and seed (creatures.generate_creature)."""

@@ -226,6 +226,17 @@ void cn_free_config(cn_config* cfg);
 cn_status cn_load_kernels(const char* path, uint32_t num_kernels, uint32_t target, float* out,
                           size_t n_floats);
 
+/* Writers for the two input files (train/generate_kernel_json.py:171-213).
+ * cn_write_kernels: K spectra of n*n complex64 as little-endian (re, im) f32 pairs, kernel-major,
+ * row-major (the .bin format read by load_kernels, config.rs:49-55); kernels_fft holds
+ * 2*K*n*n floats.
+ * cn_write_config: the creature JSON (seed_size, num_channels, num_kernels, seed_channels,
+ * c0, c1, growth_mu, growth_sigma, growth_weights).  cfg is in the loader's convention
+ * (c0 = input channel); the file gets the JSON convention (JSON c0 = cfg->c1, JSON c1 = cfg->c0)
+ * so that cn_load_config reads back cfg exactly (seeds are written with 17 significant digits). */
+cn_status cn_write_kernels(const char* path, const float* kernels_fft, uint32_t num_kernels, uint32_t n);
+cn_status cn_write_config(const char* path, const cn_config* cfg);
+
 /* ------------------------------------------------------------------------------------ */
 /* Synthetic creatures (no reference counterpart: the benchmark distributions)           */
 /* ------------------------------------------------------------------------------------ */
@@ -250,6 +261,19 @@ typedef struct cn_creature_params {
  * spatial [K][n*n] f64 kernels, state [C][n*n] f32, mu/sigma/weights [K]. */
 cn_status cn_generate_creature(const cn_creature_params* p, float* kernels_fft, double* spatial,
                                float* state, float* mu, float* sigma, float* weights);
+
+/* New (the reference generates spectra on the host, train/generate_kernel_json.py:106-145,
+ * and uploads them with set_kernel, mod.rs:207): generate creature i of an engine in place on
+ * the device from the distribution above.  Kernels are evaluated and normalised in FP64 on the
+ * grid and transformed by the engine's f32 FFT straight into its spectrum storage; the seed
+ * state (current state) and the growth parameters are bit-identical to cn_generate_creature.
+ * p->n, num_channels, num_kernels, src and dst must equal the creature's.  Synchronises. */
+cn_status cn_creature_generate(cn_engine* e, uint32_t i, const cn_creature_params* p);
+
+/* New: the kernel spectra creature i steps with, as full N*N complex64 (interleaved re, im; out
+ * holds 2*K*N*N floats), unscaled like set_kernel's input.  For spectra given to set_kernel this
+ * is their Hermitian part (the part a real convolution uses).  Synchronises. */
+cn_status cn_creature_download_kernels(cn_engine* e, uint32_t i, float* out, size_t n_floats);
 
 /* ------------------------------------------------------------------------------------ */
 /* Test hooks                                                                            */

@@ -1,5 +1,6 @@
 """Loader parity with ml-rs/config.rs on synthetic files with hand-computed expectations."""
 import json
+import struct
 
 import numpy as np
 import pytest
@@ -114,3 +115,53 @@ def test_load_kernels_rejects_bad_files(cn, tmp_path):
     with pytest.raises(cn.CellaNovaError) as ei:
         cn.load_kernels(tmp_path / "missing.bin", 1, 4)
     assert ei.value.status == 4
+
+
+# --- writers (train/generate_kernel_json.py:171-213) ------------------------------------------
+def test_write_kernels_is_the_reference_bin_format(cn, tmp_path):
+    rng = np.random.default_rng(3)
+    k = (rng.standard_normal((3, 64)) + 1j * rng.standard_normal((3, 64))).astype(np.complex64)
+    p = tmp_path / "k.bin"
+    cn.write_kernels(p, k)
+    raw = p.read_bytes()
+    assert len(raw) == 3 * 64 * 8
+    # struct.pack("ff", re, im) per value, kernel-major then row-major
+    want = b"".join(struct.pack("<ff", float(v.real), float(v.imag)) for v in k.reshape(-1))
+    assert raw == want
+    assert np.array_equal(cn.load_kernels(p, 3, 8), k)
+    padded = cn.load_kernels(p, 3, 16)            # and the loader's spectral zero-pad path reads it
+    assert padded.shape == (3, 256)
+
+
+def test_write_config_round_trip_and_json_convention(cn, tmp_path):
+    rng = np.random.default_rng(4)
+    C, K, n = 3, 5, 4
+    cfg = cn.SimConfig(seed_size=n, num_channels=C, num_kernels=K,
+                       seed_channels=[rng.random(n * n) for _ in range(C)],
+                       c0=np.array([0, 1, 2, 0, 1], np.uint32), c1=np.array([2, 2, 0, 1, 1], np.uint32),
+                       growth_mu=rng.random(K).astype(np.float32), growth_sigma=rng.random(K).astype(np.float32),
+                       growth_weights=rng.random(K).astype(np.float32))
+    p = tmp_path / "c.json"
+    cn.write_config(p, cfg)
+    doc = json.loads(p.read_text())
+    assert sorted(doc) == sorted(["seed_size", "num_channels", "num_kernels", "seed_channels", "c0", "c1",
+                                  "growth_mu", "growth_sigma", "growth_weights"])
+    assert doc["c0"] == cfg.c1.tolist() and doc["c1"] == cfg.c0.tolist()   # JSON convention (config.rs:40)
+    assert np.array_equal(np.array(doc["seed_channels"]), np.array(cfg.seed_channels))  # 17 digits: exact
+    back = cn.load_config(p)
+    assert back.seed_size == n and back.num_channels == C and back.num_kernels == K
+    assert np.array_equal(back.c0, cfg.c0) and np.array_equal(back.c1, cfg.c1)
+    for a, b in ((back.growth_mu, cfg.growth_mu), (back.growth_sigma, cfg.growth_sigma),
+                 (back.growth_weights, cfg.growth_weights)):
+        assert np.array_equal(a, b)
+    assert all(np.array_equal(a, b) for a, b in zip(back.seed_channels, cfg.seed_channels))
+
+
+def test_writers_reject_bad_arguments(cn, tmp_path):
+    with pytest.raises(cn.CellaNovaError):
+        cn.write_kernels(tmp_path / "no_dir" / "k.bin", np.zeros((1, 16), np.complex64))
+    cfg = cn.SimConfig(seed_size=4, num_channels=1, num_kernels=1, seed_channels=[np.full(16, np.nan)],
+                       c0=np.zeros(1, np.uint32), c1=np.zeros(1, np.uint32), growth_mu=np.ones(1, np.float32),
+                       growth_sigma=np.ones(1, np.float32), growth_weights=np.ones(1, np.float32))
+    with pytest.raises(cn.CellaNovaError, match="not finite"):
+        cn.write_config(tmp_path / "c.json", cfg)
