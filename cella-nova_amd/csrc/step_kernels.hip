@@ -200,23 +200,21 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
 // ---------------------------------------------------------------------------------------
 // Diffusion: Z = box3(A) (periodic), new = A * box3(state / Z).  Halo-2 tile in LDS.
 //
-// Mass conservation in FP32.  Exactly, sum_p new_p = sum_n (s_n / Z_n) * sum_{p in N(n)} A_p
+// Mass conservation.  Exactly, sum_p new_p = sum_n (s_n / Z_n) * sum_{p in N(n)} A_p
 // = sum_n s_n.  Plain FP32 breaks the identity by a systematic ~5e-8 per step (the reference's
 // f32 CPU step drifts the same way): Z_n is not exactly the sum of the A_p that later receive
 // q_n, and in smooth regions these roundings repeat with the same sign.  Here Z and the gather
-// use error-free TwoSum chains (hi + lo) in the reference's summation order (the high part is
-// the reference's plain sum), each site keeps the exact residual r_n = s_n - q_n*(Z_hi + Z_lo)
-// of its division (FMA) and adds it back to its own output.  What is left is the final
-// rounding of each output (no systematic sign).  Measured on the benchmark creature: FP64 mass
-// drift 4e-10 over 300 steps at 2048^2 (plain FP32: 1.5e-5).
+// are FP64 sums of the f32 tile values (rounding 2^-53; FP64 adds issue at the FP32 rate, and
+// the LDS tile stays f32), q_n = s_n / Z_n is a fast f32 quotient, and each site keeps the FP64 residual
+// r_n = s_n - q_n Z_n of its division and adds it to its own output, so whatever q_n is,
+// sum_p new_p = sum_n (q_n Z_n + r_n) = sum_n s_n up to FP64 rounding.  What is left is the final
+// rounding of each output to f32 (round to nearest, no systematic sign).
+//
+// Tiles are 64 x TH; blockIdx.x enumerates the tiles of one plane and is remapped so that each
+// XCD (its own L2) gets a contiguous band of tile rows: the halo columns and rows that
+// neighbouring tiles share then come from the same L2 instead of HBM.
 // ---------------------------------------------------------------------------------------
 constexpr int kDiffTW = 64, kDiffThreads = 256;
-
-__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
-  s = a + b;
-  const float bb = s - a;
-  e = (a - (s - bb)) + (b - bb);
-}
 
 // IEEE round-to-nearest f32 square root of x in [0, 1].  The hardware square root is within
 // 1 ulp; the correctly rounded result t is the candidate in {s - ulp, s, s + ulp} whose rounding
@@ -246,23 +244,26 @@ k_diffusion(const CreatureDev* __restrict__ descs, int sin, int sout, int N, int
   constexpr int TW = kDiffTW;
   const CreatureDev& d = descs[blockIdx.z];
   if (d.N != N) return;
-  const int tiles_y = (N + TH - 1) / TH;
-  const int c = blockIdx.y / tiles_y;
+  const int c = blockIdx.y;
   if (c >= d.C) return;
-  const int y0 = (blockIdx.y % tiles_y) * TH;
-  const int x0 = blockIdx.x * TW;
+  const int tiles_x = (N + TW - 1) / TW;
+  const int ntiles = gridDim.x;
+  const int bx = (int)blockIdx.x;
+  const int tile = (ntiles % 8 == 0) ? (bx % 8) * (ntiles / 8) + bx / 8 : bx;
+  const int y0 = (tile / tiles_x) * TH;
+  const int x0 = (tile % tiles_x) * TW;
   const int mask = N - 1;
   const size_t plane = (size_t)N * N;
   const float* A = d.A + c * plane;
   const float* s = slot_ptr(d, sin) + c * plane;
   float* o = slot_ptr(d, sout) + c * plane;
-
   // Render epilogue (render.rs:145-153, compute_1024.wgsl:429-443): channel c < 3 writes its
   // byte of the packed 0x00RRGGBB pixel (little endian: R is byte 2, B byte 0).
   unsigned char* rgb = (render && c < 3 && d.render) ? reinterpret_cast<unsigned char*>(d.render) : nullptr;
+
   __shared__ float sA[TH + 4][TW + 5];
   __shared__ float sQ[TH + 2][TW + 3];
-  __shared__ float sR[TH + 2][TW + 3];
+  __shared__ float sR[TH][TW];
   const int tid = threadIdx.x;
   for (int i = tid; i < (TH + 4) * (TW + 4); i += kDiffThreads) {
     const int r = i / (TW + 4), cc = i % (TW + 4);
@@ -272,34 +273,23 @@ k_diffusion(const CreatureDev* __restrict__ descs, int sin, int sout, int N, int
   __syncthreads();
   for (int i = tid; i < (TH + 2) * (TW + 2); i += kDiffThreads) {
     const int r = i / (TW + 2), cc = i % (TW + 2);
-    float zh = sA[r][cc], zl = 0.f;
-#pragma unroll
-    for (int j = 1; j < 9; ++j) {
-      float e;
-      two_sum(zh, sA[r + j / 3][cc + j % 3], zh, e);
-      zl += e;
-    }
     const int gy = (y0 - 1 + r) & mask, gx = (x0 - 1 + cc) & mask;
     const float sv = s[(size_t)gy * N + gx];
-    const float q = sv / zh;
+    double z = (double)sA[r][cc];
+#pragma unroll
+    for (int j = 1; j < 9; ++j) z += (double)sA[r + j / 3][cc + j % 3];
+    const float q = sv * __builtin_amdgcn_rcpf((float)z);
     sQ[r][cc] = q;
-    sR[r][cc] = fmaf(-q, zl, fmaf(-q, zh, sv));  // s - q*(zh + zl)
+    if (r >= 1 && r <= TH && cc >= 1 && cc <= TW) sR[r - 1][cc - 1] = (float)fma(-(double)q, z, (double)sv);
   }
   __syncthreads();
   for (int i = tid; i < TH * TW; i += kDiffThreads) {
     const int r = i / TW, cc = i % TW;
     const int gy = y0 + r, gx = x0 + cc;
-    float sh = sQ[r][cc], sl = 0.f;
+    double g = (double)sQ[r][cc];
 #pragma unroll
-    for (int j = 1; j < 9; ++j) {
-      float e;
-      two_sum(sh, sQ[r + j / 3][cc + j % 3], sh, e);
-      sl += e;
-    }
-    const float a = sA[r + 2][cc + 2];
-    const float t = a * sh;
-    const float err = fmaf(a, sh, -t);
-    const float v = t + (err + fmaf(a, sl, sR[r + 1][cc + 1]));
+    for (int j = 1; j < 9; ++j) g += (double)sQ[r + j / 3][cc + j % 3];
+    const float v = (float)fma((double)sA[r + 2][cc + 2], g, (double)sR[r][cc]);
     if (gy < N && gx < N) {
       o[(size_t)gy * N + gx] = v;
       if (rgb) rgb[((size_t)gy * N + gx) * 4 + (2 - c)] = render_byte(v);
@@ -419,7 +409,7 @@ static hipError_t launch_step_n(const StepLaunch& sl, hipStream_t st) {
   if (!sl.skip_diffusion) {
     const int th = sl.diff_th ? sl.diff_th : DefaultGeom<N>::DIFF_TH;
     const int tiles_x = (N + kDiffTW - 1) / kDiffTW, tiles_y = (N + th - 1) / th;
-    const dim3 g_dif(tiles_x, tiles_y * sl.max_c, sl.count);
+    const dim3 g_dif(tiles_x * tiles_y, sl.max_c, sl.count);
     switch (th) {
       case 8: k_diffusion<8><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N, sl.render); break;
       case 16: k_diffusion<16><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N, sl.render); break;
