@@ -64,7 +64,69 @@ def _sanitize(mu, sigma, w):
     return np.clip(mu, 0.0, 2.0), np.maximum(sigma, 1e-3), np.maximum(w, 0.0)
 
 
-def batched_grad(pkg, count=32, n=512, C=5, T=32, ckpt=4, iters=50, plain_iters=None, seed=0):
+def torch_reference_rate(crs, targets, T, iters, device="cuda:0"):
+    """The reference's own GPU gradient path, one creature at a time: the PyTorch DiffusionLenia
+    step (train/diff_lenia_org.py:27-53 with MCLenia.get_fftconv / growth, lenia_org.py:232-275:
+    dense (C, C) kernel broadcast, F.pad circular + F.unfold 3x3 sums) in fp32, autograd through
+    T steps (no checkpointing, as in the reference), MSE to the target, torch.optim.Adam on
+    mu/sigma/weights.  Returns (grad-steps/s, mean final loss)."""
+    import torch
+    import torch.nn.functional as F
+
+    def step(state, kd, mu, sigma, w):
+        C, H, W = state.shape
+        S = torch.fft.fft2(state)[:, None]                       # (C_src, 1, H, W)
+        U = torch.fft.ifft2(S * kd).real                          # (C_src, C_dst, H, W)
+        G = 2 * torch.exp(-((U - mu[..., None, None]) ** 2 / sigma[..., None, None] ** 2) / 2) - 1
+        A = torch.exp((G * w[..., None, None]).sum(dim=0))[None]  # (1, C, H, W), temp = 1
+        Z = F.unfold(F.pad(A, (1, 1, 1, 1), mode="circular"), kernel_size=(3, 3)).reshape(1, C, 9, H, W).sum(2)
+        q = F.pad(state[None] / Z, (1, 1, 1, 1), mode="circular")
+        q = F.unfold(q, kernel_size=(3, 3)).reshape(1, C, 9, H, W)
+        return (A[:, :, None] * q).sum(dim=2)[0]
+
+    def dense(cr, key):
+        C = cr["num_channels"]
+        a = np.zeros((C, C), np.float32)
+        a[cr["src"], cr["dst"]] = cr[key]
+        return torch.tensor(a, device=device)
+
+    setups = []
+    for cr, tg in zip(crs, targets):
+        C, n = cr["num_channels"], cr["n"]
+        kd = np.zeros((C, C, n, n), np.complex64)
+        kd[cr["src"], cr["dst"]] = np.asarray(cr["kernels_fft"]).reshape(-1, n, n)
+        params = [dense(cr, k).requires_grad_(True) for k in ("mu", "sigma", "weights")]
+        setups.append((torch.tensor(cr["state"], device=device), torch.tensor(kd, device=device),
+                       torch.tensor(np.asarray(tg).reshape(C, n, n), device=device), params,
+                       torch.optim.Adam(params, lr=5e-3)))
+    losses = []
+    t0 = None
+    for it in range(iters + 1):  # iteration 0 warms up cuFFT plans and the allocator (untimed)
+        if it == 1:
+            if device.startswith("cuda"):
+                torch.cuda.synchronize()
+            t0 = time.perf_counter()
+        for s0, kd, tgt, params, opt in setups:
+            opt.zero_grad()
+            s = s0
+            for _ in range(T):
+                s = step(s, kd, *params)
+            loss = ((s - tgt) ** 2).mean()
+            loss.backward()
+            opt.step()
+            with torch.no_grad():
+                params[0].clamp_(0.0, 2.0)
+                params[1].clamp_(min=1e-3)
+                params[2].clamp_(min=0.0)
+            if it == iters:
+                losses.append(float(loss.detach()))
+    if device.startswith("cuda"):
+        torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return len(setups) * T * iters / dt, float(np.mean(losses))
+
+
+def batched_grad(pkg, count=32, n=512, C=5, T=32, ckpt=4, iters=50, plain_iters=None, seed=0, torch_iters=2):
     """Batched gradient-based creature search: `count` creatures (C channels, K = C^2 kernels,
     radii U{10..18}), T-step rollouts with a checkpoint every `ckpt` steps, MSE to the state a
     second seeded creature reaches after T steps, Adam on mu/sigma/weights.  Rate in
@@ -122,11 +184,16 @@ def batched_grad(pkg, count=32, n=512, C=5, T=32, ckpt=4, iters=50, plain_iters=
         e.close()
     rate_b = count * T * iters / dt_b
     rate_p = count * T * pi / dt_p
-    return {"config": f"{count} creatures {n}x{n} C{C}K{C * C} R~U{{10..18}}, T={T}, ckpt {ckpt}, Adam on "
-                      f"mu/sigma/w, {iters} iterations (plain loop: {pi})",
-            "grad_steps_per_s_batched": round(rate_b, 1), "grad_steps_per_s_plain": round(rate_p, 1),
-            "ratio": round(rate_b / rate_p, 2), "final_mse_mean_batched": float(np.mean(loss_b)),
-            "final_mse_mean_plain": float(np.mean(loss_p))}
+    out = {"config": f"{count} creatures {n}x{n} C{C}K{C * C} R~U{{10..18}}, T={T}, ckpt {ckpt}, Adam on "
+                     f"mu/sigma/w, {iters} iterations (plain loop: {pi}, PyTorch reference path: {torch_iters})",
+           "grad_steps_per_s_batched": round(rate_b, 1), "grad_steps_per_s_plain": round(rate_p, 1),
+           "ratio": round(rate_b / rate_p, 2), "final_mse_mean_batched": float(np.mean(loss_b)),
+           "final_mse_mean_plain": float(np.mean(loss_p))}
+    if torch_iters:
+        rate_t, loss_t = torch_reference_rate(crs, targets, T, torch_iters)
+        out.update({"grad_steps_per_s_torch_reference": round(rate_t, 1), "ratio_vs_torch_reference":
+                    round(rate_b / rate_t, 2), "final_mse_mean_torch_reference": loss_t})
+    return out
 
 
 def batched_forward(pkg, steps=200, seed=0):
@@ -192,6 +259,8 @@ def main():
                     help="step2048: the headline 2048^2 C3K9 step; the batched workloads print their own line")
     ap.add_argument("--iters", type=int, default=50, help="batched-grad: Adam iterations")
     ap.add_argument("--plain-iters", type=int, default=0, help="batched-grad: iterations of the plain loop")
+    ap.add_argument("--torch-iters", type=int, default=2,
+                    help="batched-grad: iterations of the PyTorch reference path (0: skip)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a device-to-device copy of this many MiB (PMC calibration only)")
@@ -203,7 +272,7 @@ def main():
 
     pkg = ge.load_package()
     if args.workload == "batched-grad":
-        print(json.dumps({"workload": "batched-grad", **batched_grad(pkg, iters=args.iters,
+        print(json.dumps({"workload": "batched-grad", **batched_grad(pkg, iters=args.iters, torch_iters=args.torch_iters,
                                                                       plain_iters=args.plain_iters or None)}),
               flush=True)
         return

@@ -144,7 +144,7 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
 // Row inverse (C2R) of every kernel's result + growth + weighted sum + affinity exp.
 // ---------------------------------------------------------------------------------------
 template <int N, int ER, bool WRITE_U>
-__global__ void __launch_bounds__((RowCfg<N, ER>::THREADS)) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__((RowCfg<N, ER>::THREADS)) __attribute__((amdgpu_waves_per_eu(WRITE_U ? 4 : 6)))
 k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) {
   using CF = RowCfg<N, ER>;
   constexpr int L = CF::L, E = CF::E, T = CF::T, ROWS = CF::ROWS;
