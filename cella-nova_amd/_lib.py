@@ -128,6 +128,14 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with `make -C cella-nova_amd` or __graft_entry__.build()")
+        # One HIP runtime per process: PyTorch's ROCm wheel carries its own libamdhip64 (same
+        # soname).  Loading torch first lets this library bind to that copy, so device pointers
+        # and streams are shared with torch; loaded the other way round, torch would load a
+        # second runtime and fail to find the GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         h = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(h, name)
