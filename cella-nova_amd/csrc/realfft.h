@@ -19,21 +19,26 @@ __device__ __forceinline__ void r2c_finish(const float2* v, float2* lds, int t, 
 #pragma unroll
   for (int i = 0; i < E; ++i) lds[LdsMap<L, ROWS, false>::at(out_pos<L, E, false>(t, i), lane)] = v[i];
   __syncthreads();
+  // Branch-free: every lane loads its partner and twiddle (p = 0 reads itself and tw[0]) and
+  // the packed (X[0], X[N/2]) slot is selected at the end, so no load sits in a divergent
+  // branch with its own wait.
+  float2 w[E], zq[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int p = out_pos<L, E, false>(t, i);
+    w[i] = tw[p * (kTwN / N)];
+    zq[i] = lds[LdsMap<L, ROWS, false>::at((L - p) & (L - 1), lane)];
+  }
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const int p = out_pos<L, E, false>(t, i);
     const float2 zp = v[i];
-    float2 X;
-    if (p == 0) {
-      X = make_float2(zp.x + zp.y, zp.x - zp.y);
-    } else {
-      const float2 zq = lds[LdsMap<L, ROWS, false>::at(L - p, lane)];
-      const float2 ze = make_float2(0.5f * (zp.x + zq.x), 0.5f * (zp.y - zq.y));
-      const float2 dd = make_float2(zp.x - zq.x, zp.y + zq.y);  // zp - conj(zq)
-      const float2 zo = make_float2(0.5f * dd.y, -0.5f * dd.x);  // dd / (2i)
-      X = c_add(ze, c_mul(tw[p * (kTwN / N)], zo));
-    }
-    if (store) out[p] = X;
+    const float2 ze = make_float2(0.5f * (zp.x + zq[i].x), 0.5f * (zp.y - zq[i].y));
+    const float2 dd = make_float2(zp.x - zq[i].x, zp.y + zq[i].y);  // zp - conj(zq)
+    const float2 zo = make_float2(0.5f * dd.y, -0.5f * dd.x);        // dd / (2i)
+    const float2 X = c_add(ze, c_mul(w[i], zo));
+    const float2 X0 = make_float2(zp.x + zp.y, zp.x - zp.y);
+    if (store) out[p] = (p == 0) ? X0 : X;
   }
 }
 
@@ -57,17 +62,16 @@ template <int N, int E>
 __device__ __forceinline__ void c2r_form(const float2* xp, const float2* xq, int t, const float2* __restrict__ tw,
                                          float2* v) {
   constexpr int L = N / 2;
+  // Branch-free (see r2c_finish): the twiddle is loaded for every lane, slot 0 selected last.
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const int p = in_pos<L, E, true>(t, i);
-    if (p == 0) {
-      v[i] = make_float2(xp[i].x + xp[i].y, xp[i].x - xp[i].y);
-    } else {
-      const float2 e2 = make_float2(xp[i].x + xq[i].x, xp[i].y - xq[i].y);  // xp + conj(xq)
-      const float2 dd = make_float2(xp[i].x - xq[i].x, xp[i].y + xq[i].y);  // xp - conj(xq)
-      const float2 o2 = c_mulc(dd, tw[p * (kTwN / N)]);                      // * W_N^{-p}
-      v[i] = make_float2(e2.x - o2.y, e2.y + o2.x);                         // e2 + i*o2
-    }
+    const float2 e2 = make_float2(xp[i].x + xq[i].x, xp[i].y - xq[i].y);  // xp + conj(xq)
+    const float2 dd = make_float2(xp[i].x - xq[i].x, xp[i].y + xq[i].y);  // xp - conj(xq)
+    const float2 o2 = c_mulc(dd, tw[p * (kTwN / N)]);                      // * W_N^{-p}
+    const float2 g = make_float2(e2.x - o2.y, e2.y + o2.x);               // e2 + i*o2
+    const float2 g0 = make_float2(xp[i].x + xp[i].y, xp[i].x - xp[i].y);
+    v[i] = (p == 0) ? g0 : g;
   }
 }
 

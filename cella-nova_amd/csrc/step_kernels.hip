@@ -105,12 +105,13 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
   for (int kk = kb; kk < ke; ++kk) {
     const int k = d.src_k[kk];
     t = launder(t);
-    const float2* kh = khalf + (size_t)k * N * HALF;
     float2 w[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) w[i] = c_mul(v[i], nk[i]);
     if (kk + 1 < ke) load_kernel(d.src_k[kk + 1], nk);
     if (edge) {
+      // Lane 0 splits the packed DC/Nyquist column; its kernel factors (khalf column 0, knyq)
+      // are loaded a few at a time so that the loads overlap.
       __syncthreads();
       if (lane == 0) {
 #pragma unroll
@@ -118,18 +119,31 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
       }
       __syncthreads();
       if (lane == 0) {
+        const float2* kh = khalf + (size_t)k * N * HALF;
         const float2* kn = reinterpret_cast<const float2*>(d.knyq) + (size_t)k * N;
+        constexpr int G4 = 2;
 #pragma unroll
-        for (int i = 0; i < E; ++i) {
-          const int y = out_pos<L, E, false>(t, i);
-          const float2 F = v[i];
-          const float2 G = lds[LdsMap<L, TC, true>::at((N - y) & (N - 1), 0)];
-          const float2 a = make_float2(0.5f * (F.x + G.x), 0.5f * (F.y - G.y));  // (F + conj G)/2
-          const float2 dd = make_float2(F.x - G.x, F.y + G.y);                    // F - conj G
-          const float2 b = make_float2(0.5f * dd.y, -0.5f * dd.x);                // dd / (2i)
-          const float2 pa = c_mul(a, kh[(size_t)y * HALF]);
-          const float2 pb = c_mul(b, kn[y]);
-          w[i] = make_float2(pa.x - pb.y, pa.y + pb.x);  // pa + i*pb
+        for (int i0 = 0; i0 < E; i0 += G4) {
+          float2 kd[G4], kq[G4];
+#pragma unroll
+          for (int u = 0; u < G4; ++u) {
+            const int y = out_pos<L, E, false>(t, i0 + u);
+            kd[u] = kh[(size_t)y * HALF];
+            kq[u] = kn[y];
+          }
+#pragma unroll
+          for (int u = 0; u < G4; ++u) {
+            const int i = i0 + u;
+            const int y = out_pos<L, E, false>(t, i);
+            const float2 F = v[i];
+            const float2 G = lds[LdsMap<L, TC, true>::at((N - y) & (N - 1), 0)];
+            const float2 a = make_float2(0.5f * (F.x + G.x), 0.5f * (F.y - G.y));  // (F + conj G)/2
+            const float2 dd = make_float2(F.x - G.x, F.y + G.y);                    // F - conj G
+            const float2 b = make_float2(0.5f * dd.y, -0.5f * dd.x);                // dd / (2i)
+            const float2 pa = c_mul(a, kd[u]);
+            const float2 pb = c_mul(b, kq[u]);
+            w[i] = make_float2(pa.x - pb.y, pa.y + pb.x);  // pa + i*pb
+          }
         }
       }
     }
@@ -144,7 +158,7 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
 // Row inverse (C2R) of every kernel's result + growth + weighted sum + affinity exp.
 // ---------------------------------------------------------------------------------------
 template <int N, int ER, bool WRITE_U>
-__global__ void __launch_bounds__((RowCfg<N, ER>::THREADS)) __attribute__((amdgpu_waves_per_eu(WRITE_U ? 4 : 6)))
+__global__ void __launch_bounds__((RowCfg<N, ER>::THREADS)) __attribute__((amdgpu_waves_per_eu(4)))
 k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) {
   using CF = RowCfg<N, ER>;
   constexpr int L = CF::L, E = CF::E, T = CF::T, ROWS = CF::ROWS;
