@@ -380,13 +380,12 @@ static hipError_t launch_adjoint_n(const StepLaunch& sl, int gin, int gout, hipS
     }
   };
   if ((e = rows(true)) != hipSuccess) return e;
-  const int tc = sl.col_tc ? sl.col_tc : DefaultGeom<N>::TC;
-  switch (sl.col_e ? sl.col_e : DefaultGeom<N>::COL_E) {
-    case 8: e = launch_col_adj_tc<N, 8>(sl, tc, st); break;
-    case 16: e = launch_col_adj_tc<N, 16>(sl, tc, st); break;
-    default: e = hipErrorInvalidValue;
-  }
-  if (e != hipSuccess) return e;
+  // The adjoint column kernel keeps a per-source accumulator beside the transform, so it runs
+  // with 8 values per thread (16 would spill), on the plan's tile width or the widest valid one
+  // below it.
+  int tc = sl.col_tc ? sl.col_tc : DefaultGeom<N>::TC;
+  while (tc > 2 && tc * ColCfg<N, 8>::T > 1024) tc /= 2;
+  if ((e = launch_col_adj_tc<N, 8>(sl, tc, st)) != hipSuccess) return e;
   if ((e = rows(false)) != hipSuccess) return e;
   return hipGetLastError();
 }
