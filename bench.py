@@ -262,6 +262,7 @@ def main():
     ap.add_argument("--torch-iters", type=int, default=2,
                     help="batched-grad: iterations of the PyTorch reference path (0: skip)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--plan", default="", help="fixed launch plan fwd_e,inv_e,col_e,col_tc,diff_th (no autotune)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a device-to-device copy of this many MiB (PMC calibration only)")
     args = ap.parse_args()
@@ -290,7 +291,12 @@ def main():
     cr = pkg.baseline_creature(n, num_channels=C, radius=R, seed=0)
     K = cr["num_kernels"]
     eng = pkg.make_engine(cr)
-    plan = eng.plans()[0] if args.no_autotune else eng.autotune(5)[0]
+    if args.plan:
+        fe, ie, ce, tc, th = (int(v) for v in args.plan.split(","))
+        eng.set_plan(n, fe, ie, ce, tc, th)
+        plan = eng.plans()[0]
+    else:
+        plan = eng.plans()[0] if args.no_autotune else eng.autotune(5)[0]
     try:
         import torch
         sync_dev = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
