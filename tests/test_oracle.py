@@ -250,3 +250,19 @@ def test_render_rgb_fewer_channels_and_order(orc):
     assert (rgb[..., 0] == 220).all() and (rgb[..., 1:] == 0).all()
     st5 = np.stack([np.full((4, 4), v, np.float32) for v in (0.0, 0.5, 2.0 / 3.0, 1.0, 1.0)])
     assert orc.render_rgb(st5)[0, 0].tolist() == [0, 220, 255]   # channels 3.. are not drawn
+
+
+# --- the PyTorch restatement (oracle/torch_ref.py) against the C++ FP64 oracle -----------------
+@pytest.mark.parametrize("C,n", [(1, 32), (2, 32), (3, 64)])
+def test_torch_restatement_matches_fp64_oracle(cn, orc, C, n):
+    torch = pytest.importorskip("torch")
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+    import torch_ref
+    cr = cn.baseline_creature(n, num_channels=C, radius=max(3, n // 6), seed=C, patch=n // 2)
+    want = orc.run_f64(cr["state"].astype(np.float64), cr["kernels_fft"], cr["src"], cr["dst"], cr["mu"],
+                       cr["sigma"], cr["weights"], temp=0.7)
+    got = torch_ref.step(torch.tensor(cr["state"], dtype=torch.float64), *torch_ref.creature_tensors(cr, "cpu"),
+                         temp=float(np.float32(0.7))).numpy()  # the C ABI takes temp as f32
+    assert orc.rel_linf(got, want) <= 1e-12
