@@ -173,6 +173,10 @@ struct Plan {
 // touch consecutive addresses; used for column tiles).  Otherwise lane*LP + pos + pad.
 // pad = pos/16 inserts one element every 16: the first pass of a transform writes its outputs
 // at stride R (16), which without padding puts every lane of a wave on the same LDS banks.
+//
+// at(a + c) == at(a) + off(c) whenever (a mod 16) + (c mod 16) < 16; callers use this with a
+// compile-time c so that the padded address is one per-butterfly base plus the LDS
+// instruction's immediate offset (no per-element index arithmetic).
 template <int L, int NT, bool INTERLEAVED>
 struct LdsMap {
   static constexpr int LP = L + L / 16;
@@ -181,6 +185,7 @@ struct LdsMap {
     const int pp = pos + (pos >> 4);
     return INTERLEAVED ? pp * NT + lane : lane * LP + pp;
   }
+  static constexpr int off(int c) { return INTERLEAVED ? (c + c / 16) * NT : c + c / 16; }
 };
 
 // One Stockham pass on register data (twiddle + radix-R butterflies) for thread t.
@@ -227,21 +232,39 @@ __device__ __forceinline__ void fft_run(float2* v, float2* lds, int t, int lane,
   if constexpr (P + 1 < PL::npass) {
     constexpr int T = L / E;
     constexpr int R2 = PL::radix(P + 1, INV);
+    using LM = LdsMap<L, NT, IL>;
+    // Writes go to base + q*NS.  base mod 16 is k < NS when NS*R >= 16 (base - k is a multiple
+    // of NS*R), a multiple of R when NS == 1, and q*NS is a multiple of 16 when NS >= 16: in
+    // each case the padded index splits into at(base) + off(q*NS).  Reads go to j + m*L/R2.
+    constexpr bool kSplitW = (NS % 16 == 0) || (NS == 1 && 16 % R == 0) || (16 % NS == 0 && NS * R >= 16);
+    constexpr bool kSplitR = (L / R2) % 16 == 0;
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < E / R; ++b) {
       const int j = t + b * T;
       const int k = j & (NS - 1);
       const int base = (j - k) * R + k;
+      if constexpr (kSplitW) {
+        float2* p = lds + LM::at(base, lane);
 #pragma unroll
-      for (int q = 0; q < R; ++q) lds[LdsMap<L, NT, IL>::at(base + q * NS, lane)] = v[b * R + q];
+        for (int q = 0; q < R; ++q) p[LM::off(q * NS)] = v[b * R + q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < R; ++q) lds[LM::at(base + q * NS, lane)] = v[b * R + q];
+      }
     }
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < E / R2; ++b) {
       const int j = t + b * T;
+      if constexpr (kSplitR) {
+        const float2* p = lds + LM::at(j, lane);
 #pragma unroll
-      for (int m = 0; m < R2; ++m) v[b * R2 + m] = lds[LdsMap<L, NT, IL>::at(j + m * (L / R2), lane)];
+        for (int m = 0; m < R2; ++m) v[b * R2 + m] = p[LM::off(m * (L / R2))];
+      } else {
+#pragma unroll
+        for (int m = 0; m < R2; ++m) v[b * R2 + m] = lds[LM::at(j + m * (L / R2), lane)];
+      }
     }
     fft_run<L, E, NT, IL, INV, TWN, P + 1>(v, lds, t, lane, tw);
   }

@@ -15,9 +15,17 @@ template <int N, int E, int ROWS>
 __device__ __forceinline__ void r2c_finish(const float2* v, float2* lds, int t, int lane,
                                            const float2* __restrict__ tw, float2* out, bool store) {
   constexpr int L = N / 2;
+  using LM = LdsMap<L, ROWS, false>;
   __syncthreads();
+  if constexpr ((L / E) % 16 == 0 && (L / Plan<L, E>::last_radix(false)) % 16 == 0) {
+    // out_pos(t, i) = t + c_i with c_i a multiple of 16: base + immediate offsets.
+    float2* p = lds + LM::at(t, lane);
 #pragma unroll
-  for (int i = 0; i < E; ++i) lds[LdsMap<L, ROWS, false>::at(out_pos<L, E, false>(t, i), lane)] = v[i];
+    for (int i = 0; i < E; ++i) p[LM::off(out_pos<L, E, false>(0, i))] = v[i];
+  } else {
+#pragma unroll
+    for (int i = 0; i < E; ++i) lds[LM::at(out_pos<L, E, false>(t, i), lane)] = v[i];
+  }
   __syncthreads();
   // Branch-free: every lane loads its partner and twiddle (p = 0 reads itself and tw[0]) and
   // the packed (X[0], X[N/2]) slot is selected at the end, so no load sits in a divergent
