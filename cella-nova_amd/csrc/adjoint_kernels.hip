@@ -135,8 +135,8 @@ k_growth_adj_row(const CreatureDev* __restrict__ descs, const float2* __restrict
   red[0][threadIdx.x] = smu;
   red[1][threadIdx.x] = ssg;
   red[2][threadIdx.x] = sw;
-  fft_run<L, E, ROWS, false, false, kTwN>(v, lds, t, lane, tw);  // barriers inside cover red[]
-  __syncthreads();
+  fft_run<L, E, ROWS, false, false, kTwN>(v, lds, t, lane, tw);
+  __syncthreads();  // orders the red[] writes above before the reduction
   for (int h = NT / 2; h > 0; h >>= 1) {
     if (threadIdx.x < h)
 #pragma unroll

@@ -209,6 +209,26 @@ __device__ __forceinline__ void pass_compute(float2* v, int t, const float2* __r
   }
 }
 
+// Orders the LDS exchange of a transform.  WAVE: all threads of each transform belong to one
+// wavefront, and a wavefront performs its LDS accesses in program order, so the compiler only
+// has to keep the accesses in order (the __syncwarp sequence); otherwise a block barrier.
+template <bool WAVE>
+__device__ __forceinline__ void xsync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+// Row-layout transforms (thread = lane*T + t) whose T threads divide a wavefront.
+template <int L, int E>
+constexpr bool wave_rows() {
+  return 64 % (L / E) == 0;
+}
+
 // Hides the value of a lane index from the optimiser.  Index arithmetic derived from a
 // laundered value is recomputed where it is used instead of being hoisted out of the
 // caller's loops as dozens of loop-invariant 64-bit addresses (which spill).
@@ -219,8 +239,9 @@ __device__ __forceinline__ int launder(int x) {
 
 // Runs passes P..npass-1 of an L-point transform.  On entry v holds the inputs of pass P
 // (layout of pass P); on exit it holds the outputs of the last pass.  `lds` must hold
-// LdsMap<L,NT,IL>::size elements; every exchange is bracketed by block barriers, so the
-// routine can be called repeatedly on the same buffer.
+// LdsMap<L,NT,IL>::size elements; every exchange is bracketed by barriers (wavefront-level for
+// row-layout transforms that fit in a wavefront, block-level otherwise), so the routine can be
+// called repeatedly on the same buffer.
 template <int L, int E, int NT, bool IL, bool INV, int TWN, int P = 0>
 __device__ __forceinline__ void fft_run(float2* v, float2* lds, int t, int lane,
                                         const float2* __restrict__ tw) {
@@ -238,7 +259,8 @@ __device__ __forceinline__ void fft_run(float2* v, float2* lds, int t, int lane,
     // each case the padded index splits into at(base) + off(q*NS).  Reads go to j + m*L/R2.
     constexpr bool kSplitW = (NS % 16 == 0) || (NS == 1 && 16 % R == 0) || (16 % NS == 0 && NS * R >= 16);
     constexpr bool kSplitR = (L / R2) % 16 == 0;
-    __syncthreads();
+    constexpr bool kWave = !IL && wave_rows<L, E>();
+    xsync<kWave>();
 #pragma unroll
     for (int b = 0; b < E / R; ++b) {
       const int j = t + b * T;
@@ -253,7 +275,7 @@ __device__ __forceinline__ void fft_run(float2* v, float2* lds, int t, int lane,
         for (int q = 0; q < R; ++q) lds[LM::at(base + q * NS, lane)] = v[b * R + q];
       }
     }
-    __syncthreads();
+    xsync<kWave>();
 #pragma unroll
     for (int b = 0; b < E / R2; ++b) {
       const int j = t + b * T;

@@ -16,7 +16,8 @@ __device__ __forceinline__ void r2c_finish(const float2* v, float2* lds, int t, 
                                            const float2* __restrict__ tw, float2* out, bool store) {
   constexpr int L = N / 2;
   using LM = LdsMap<L, ROWS, false>;
-  __syncthreads();
+  constexpr bool kWave = wave_rows<L, E>();
+  xsync<kWave>();
   if constexpr ((L / E) % 16 == 0 && (L / Plan<L, E>::last_radix(false)) % 16 == 0) {
     // out_pos(t, i) = t + c_i with c_i a multiple of 16: base + immediate offsets.
     float2* p = lds + LM::at(t, lane);
@@ -26,7 +27,7 @@ __device__ __forceinline__ void r2c_finish(const float2* v, float2* lds, int t, 
 #pragma unroll
     for (int i = 0; i < E; ++i) lds[LM::at(out_pos<L, E, false>(t, i), lane)] = v[i];
   }
-  __syncthreads();
+  xsync<kWave>();
   // Branch-free: every lane loads its partner and twiddle (p = 0 reads itself and tw[0]) and
   // the packed (X[0], X[N/2]) slot is selected at the end, so no load sits in a divergent
   // branch with its own wait.
@@ -81,6 +82,29 @@ __device__ __forceinline__ void c2r_form(const float2* xp, const float2* xq, int
     const float2 g0 = make_float2(xp[i].x + xp[i].y, xp[i].x - xp[i].y);
     v[i] = (p == 0) ? g0 : g;
   }
+}
+
+// Single-wavefront rows (wave_rows<N/2, E>()): each lane loads only its own X[p] and takes the
+// partner X[L-p] from the other lanes through `lds` (the transform's exchange buffer), ordered
+// at wavefront level.  Halves the row's global loads against c2r_load.
+template <int N, int E>
+__device__ __forceinline__ void c2r_load_own(const float2* __restrict__ X, int t, float2* xp) {
+  constexpr int L = N / 2;
+#pragma unroll
+  for (int i = 0; i < E; ++i) xp[i] = X[in_pos<L, E, true>(t, i)];
+}
+
+template <int N, int E, int ROWS>
+__device__ __forceinline__ void c2r_partner(const float2* xp, float2* lds, int t, int lane, float2* xq) {
+  constexpr int L = N / 2;
+  static_assert(wave_rows<L, E>(), "partner exchange through LDS needs one wavefront per row");
+  using LM = LdsMap<L, ROWS, false>;
+  xsync<true>();
+#pragma unroll
+  for (int i = 0; i < E; ++i) lds[LM::at(in_pos<L, E, true>(t, i), lane)] = xp[i];
+  xsync<true>();
+#pragma unroll
+  for (int i = 0; i < E; ++i) xq[i] = lds[LM::at((L - in_pos<L, E, true>(t, i)) & (L - 1), lane)];
 }
 
 template <int N, int E>
