@@ -241,11 +241,15 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
 // sum_p new_p = sum_n (q_n Z_n + r_n) = sum_n s_n up to FP64 rounding.  What is left is the final
 // rounding of each output to f32 (round to nearest, no systematic sign).
 //
-// Tiles are 64 x TH; blockIdx.x enumerates the tiles of one plane and is remapped so that each
-// XCD (its own L2) gets a contiguous band of tile rows: the halo columns and rows that
-// neighbouring tiles share then come from the same L2 instead of HBM.
+// Layout: one wavefront per tile of 60 columns x TH rows, no LDS and no barriers.  Lane l holds
+// column x0 - 2 + l (64 columns: the tile plus a 2-column halo on each side).  Horizontal 3-sums
+// take the neighbours' values with DPP wave shifts; vertical 3-sums are rolling registers while
+// the wave walks down the rows (A rows y0-2 .. y0+TH+1, state rows y0-1 .. y0+TH).  Z and q are
+// valid on lanes 1..62, the gather on lanes 2..61, which store.  Tiles are dealt to blocks so
+// that each XCD (its own L2) gets a contiguous band of tiles: neighbouring tiles share halo
+// lines through the same L2.
 // ---------------------------------------------------------------------------------------
-constexpr int kDiffTW = 64, kDiffThreads = 256;
+constexpr int kDiffTW = 60, kDiffThreads = 256;  // output columns per wave, 4 waves per block
 
 // IEEE round-to-nearest f32 square root of x in [0, 1].  The hardware square root is within
 // 1 ulp; the correctly rounded result t is the candidate in {s - ulp, s, s + ulp} whose rounding
@@ -269,62 +273,71 @@ __device__ __forceinline__ unsigned char render_byte(float v) {
   return (unsigned char)(unsigned int)(sqrt_rn01(x) * 255.0f);
 }
 
+// x(l-1) + x(l) + x(l+1) in FP64 across the wavefront (lanes 0 and 63 see a 0 neighbour).
+__device__ __forceinline__ double hsum3(float x) {
+  const float lo = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, false));
+  const float hi = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, false));
+  return (double)lo + (double)x + (double)hi;
+}
+
 template <int TH>
 __global__ void __launch_bounds__(kDiffThreads)
 k_diffusion(const CreatureDev* __restrict__ descs, int sin, int sout, int N, int render) {
-  constexpr int TW = kDiffTW;
+  constexpr int OW = kDiffTW;
   const CreatureDev& d = descs[blockIdx.z];
   if (d.N != N) return;
   const int c = blockIdx.y;
   if (c >= d.C) return;
-  const int tiles_x = (N + TW - 1) / TW;
-  const int ntiles = gridDim.x;
-  const int bx = (int)blockIdx.x;
-  const int tile = (ntiles % 8 == 0) ? (bx % 8) * (ntiles / 8) + bx / 8 : bx;
-  const int y0 = (tile / tiles_x) * TH;
-  const int x0 = (tile % tiles_x) * TW;
+  const int tiles_x = (N + OW - 1) / OW, tiles_y = (N + TH - 1) / TH;
+  const int nblk = gridDim.x, bx = (int)blockIdx.x;
+  const int blk = (nblk % 8 == 0) ? (bx % 8) * (nblk / 8) + bx / 8 : bx;
+  const int tile = blk * (kDiffThreads / 64) + (int)(threadIdx.x >> 6);
+  if (tile >= tiles_x * tiles_y) return;  // a whole wavefront; the kernel has no barriers
+  const int l = threadIdx.x & 63;
+  const int y0 = (tile / tiles_x) * TH, x0 = (tile % tiles_x) * OW;
   const int mask = N - 1;
+  const int xl = x0 - 2 + l;             // this lane's column (unwrapped)
+  const int gx = xl & mask;
+  const bool store = l >= 2 && l < 2 + OW && xl < N;
   const size_t plane = (size_t)N * N;
-  const float* A = d.A + c * plane;
-  const float* s = slot_ptr(d, sin) + c * plane;
-  float* o = slot_ptr(d, sout) + c * plane;
+  const float* A = d.A + c * plane + gx;
+  const float* s = slot_ptr(d, sin) + c * plane + gx;
+  float* o = slot_ptr(d, sout) + c * plane + xl;
   // Render epilogue (render.rs:145-153, compute_1024.wgsl:429-443): channel c < 3 writes its
   // byte of the packed 0x00RRGGBB pixel (little endian: R is byte 2, B byte 0).
   unsigned char* rgb = (render && c < 3 && d.render) ? reinterpret_cast<unsigned char*>(d.render) : nullptr;
 
-  __shared__ float sA[TH + 4][TW + 5];
-  __shared__ float sQ[TH + 2][TW + 3];
-  __shared__ float sR[TH][TW];
-  const int tid = threadIdx.x;
-  for (int i = tid; i < (TH + 4) * (TW + 4); i += kDiffThreads) {
-    const int r = i / (TW + 4), cc = i % (TW + 4);
-    const int gy = (y0 - 2 + r) & mask, gx = (x0 - 2 + cc) & mask;
-    sA[r][cc] = A[(size_t)gy * N + gx];
-  }
-  __syncthreads();
-  for (int i = tid; i < (TH + 2) * (TW + 2); i += kDiffThreads) {
-    const int r = i / (TW + 2), cc = i % (TW + 2);
-    const int gy = (y0 - 1 + r) & mask, gx = (x0 - 1 + cc) & mask;
-    const float sv = s[(size_t)gy * N + gx];
-    double z = (double)sA[r][cc];
+  // Step j computes q on row y0 - 1 + j; from j = 2 on it also finishes output row y0 + j - 2.
+  double h0 = hsum3(A[(size_t)((y0 - 2) & mask) * N]);  // horizontal A sums of rows yq-1, yq
+  double h1 = hsum3(A[(size_t)((y0 - 1) & mask) * N]);
+  double g0 = 0.0, g1 = 0.0;  // horizontal q sums of the two previous q rows
+  float a0 = 0.f, a1 = 0.f;   // A of the two previous A rows loaded in the steps
+  float r1 = 0.f;             // residual of the previous q row
 #pragma unroll
-    for (int j = 1; j < 9; ++j) z += (double)sA[r + j / 3][cc + j % 3];
+  for (int j = 0; j < TH + 2; ++j) {
+    const int yq = y0 - 1 + j;
+    const float a2 = A[(size_t)((yq + 1) & mask) * N];
+    const float sv = s[(size_t)(yq & mask) * N];
+    const double h2 = hsum3(a2);
+    const double z = h0 + h1 + h2;
     const float q = sv * __builtin_amdgcn_rcpf((float)z);
-    sQ[r][cc] = q;
-    if (r >= 1 && r <= TH && cc >= 1 && cc <= TW) sR[r - 1][cc - 1] = (float)fma(-(double)q, z, (double)sv);
-  }
-  __syncthreads();
-  for (int i = tid; i < TH * TW; i += kDiffThreads) {
-    const int r = i / TW, cc = i % TW;
-    const int gy = y0 + r, gx = x0 + cc;
-    double g = (double)sQ[r][cc];
-#pragma unroll
-    for (int j = 1; j < 9; ++j) g += (double)sQ[r + j / 3][cc + j % 3];
-    const float v = (float)fma((double)sA[r + 2][cc + 2], g, (double)sR[r][cc]);
-    if (gy < N && gx < N) {
-      o[(size_t)gy * N + gx] = v;
-      if (rgb) rgb[((size_t)gy * N + gx) * 4 + (2 - c)] = render_byte(v);
+    const float r = (float)fma(-(double)q, z, (double)sv);
+    const double g2 = hsum3(q);
+    if (j >= 2) {
+      const int yo = yq - 1;
+      const float v = (float)fma((double)a0, g0 + g1 + g2, (double)r1);
+      if (store && yo < N) {
+        o[(size_t)yo * N] = v;
+        if (rgb) rgb[((size_t)yo * N + xl) * 4 + (2 - c)] = render_byte(v);
+      }
     }
+    h0 = h1;
+    h1 = h2;
+    g0 = g1;
+    g1 = g2;
+    a0 = a1;
+    a1 = a2;
+    r1 = r;
   }
 }
 
@@ -440,7 +453,8 @@ static hipError_t launch_step_n(const StepLaunch& sl, hipStream_t st) {
   if (!sl.skip_diffusion) {
     const int th = sl.diff_th ? sl.diff_th : DefaultGeom<N>::DIFF_TH;
     const int tiles_x = (N + kDiffTW - 1) / kDiffTW, tiles_y = (N + th - 1) / th;
-    const dim3 g_dif(tiles_x * tiles_y, sl.max_c, sl.count);
+    const int waves = kDiffThreads / 64;
+    const dim3 g_dif((tiles_x * tiles_y + waves - 1) / waves, sl.max_c, sl.count);
     switch (th) {
       case 8: k_diffusion<8><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N, sl.render); break;
       case 16: k_diffusion<16><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N, sl.render); break;
