@@ -24,76 +24,77 @@
 
 namespace cn {
 
-constexpr int kAdjTW = 64, kAdjTH = 16, kAdjThreads = 256;
+constexpr int kAdjTW = 60, kAdjTH = 16, kAdjThreads = 256;  // output columns / rows per wave
+
+// x(l-1) + x(l) + x(l+1) across the wavefront (DPP wave shifts; lanes 0 and 63 see a 0).
+__device__ __forceinline__ float hsum3f(float x) {
+  const float lo = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, false));
+  const float hi = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, false));
+  return lo + x + hi;
+}
 
 // ---------------------------------------------------------------------------------------
 // Diffusion adjoint: inputs A (recomputed), state slot `sin`, gradient slot `gin`;
-// outputs dsd = T1 / Z and gaff.  Halo-2 tiles of A and g in LDS.
+// outputs dsd = T1 / Z and gaff.  Same streaming layout as the forward diffusion: one
+// wavefront per 60 x kAdjTH tile, lane l on column x0 - 2 + l, horizontal 3-sums by DPP wave
+// shifts and vertical ones in rolling registers while walking down the rows.
 // ---------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(kAdjThreads)
 k_diffusion_adj(const CreatureDev* __restrict__ descs, int sin, int gin, int N) {
-  constexpr int TW = kAdjTW, TH = kAdjTH;
+  constexpr int OW = kAdjTW, TH = kAdjTH;
   const CreatureDev& d = descs[blockIdx.z];
   if (d.N != N) return;
-  const int tiles_y = (N + TH - 1) / TH;
-  const int c = blockIdx.y / tiles_y;
+  const int c = blockIdx.y;
   if (c >= d.C) return;
-  const int y0 = (blockIdx.y % tiles_y) * TH;
-  const int x0 = blockIdx.x * TW;
+  const int tiles_x = (N + OW - 1) / OW, tiles_y = (N + TH - 1) / TH;
+  const int nblk = gridDim.x, bx = (int)blockIdx.x;
+  const int blk = (nblk % 8 == 0) ? (bx % 8) * (nblk / 8) + bx / 8 : bx;
+  const int tile = blk * (kAdjThreads / 64) + (int)(threadIdx.x >> 6);
+  if (tile >= tiles_x * tiles_y) return;  // a whole wavefront; the kernel has no barriers
+  const int l = threadIdx.x & 63;
+  const int y0 = (tile / tiles_x) * TH, x0 = (tile % tiles_x) * OW;
   const int mask = N - 1;
+  const int xl = x0 - 2 + l;
+  const int gx = xl & mask;
+  const bool store = l >= 2 && l < 2 + OW && xl < N;
   const size_t plane = (size_t)N * N;
-  const float* A = d.A + c * plane;
-  const float* s = slot_ptr(d, sin) + c * plane;
-  const float* g = d.grad + ((size_t)gin * d.C + c) * plane;
-  float* dsd = d.dsd + c * plane;
-  float* gaff = d.gaff + c * plane;
+  const float* A = d.A + c * plane + gx;
+  const float* s = slot_ptr(d, sin) + c * plane + gx;
+  const float* g = d.grad + ((size_t)gin * d.C + c) * plane + gx;
+  float* dsd = d.dsd + c * plane + xl;
+  float* gaff = d.gaff + c * plane + xl;
   const float temp = d.temp;
 
-  __shared__ float sA[TH + 4][TW + 5];
-  __shared__ float sG[TH + 4][TW + 5];
-  __shared__ float sQ[TH + 2][TW + 3];
-  __shared__ float sD[TH + 2][TW + 3];
-  const int tid = threadIdx.x;
-  for (int i = tid; i < (TH + 4) * (TW + 4); i += kAdjThreads) {
-    const int r = i / (TW + 4), cc = i % (TW + 4);
-    const size_t o = (size_t)((y0 - 2 + r) & mask) * N + ((x0 - 2 + cc) & mask);
-    sA[r][cc] = A[o];
-    sG[r][cc] = g[o];
-  }
-  __syncthreads();
-  for (int i = tid; i < (TH + 2) * (TW + 2); i += kAdjThreads) {
-    const int r = i / (TW + 2), cc = i % (TW + 2);
-    float z = 0.f, t1 = 0.f;
+  // Step j handles q row yq = y0 - 1 + j; from j = 2 on it finishes gaff of row yq - 1.
+  const size_t rm2 = (size_t)((y0 - 2) & mask) * N, rm1 = (size_t)((y0 - 1) & mask) * N;
+  const float am2 = A[rm2], am1 = A[rm1];
+  float hz0 = hsum3f(am2), hz1 = hsum3f(am1);              // horizontal sums of A
+  float ht0 = hsum3f(am2 * g[rm2]), ht1 = hsum3f(am1 * g[rm1]);  // ... of A g
+  float hq0 = 0.f, hq1 = 0.f, hd0 = 0.f, hd1 = 0.f;        // ... of q and dZ (previous q rows)
+  float a0 = 0.f, a1 = 0.f, g0 = 0.f, g1 = 0.f;            // A, g of the previous step rows
 #pragma unroll
-    for (int dr = 0; dr < 3; ++dr)
-#pragma unroll
-      for (int dc = 0; dc < 3; ++dc) {
-        const float a = sA[r + dr][cc + dc];
-        z += a;
-        t1 += a * sG[r + dr][cc + dc];
-      }
-    const int gy = (y0 - 1 + r) & mask, gx = (x0 - 1 + cc) & mask;
-    const float q = s[(size_t)gy * N + gx] / z;
-    sQ[r][cc] = q;
-    sD[r][cc] = -t1 * q / z;
-    if (r >= 1 && r <= TH && cc >= 1 && cc <= TW && y0 - 1 + r < N && x0 - 1 + cc < N)
-      dsd[(size_t)gy * N + gx] = t1 / z;
-  }
-  __syncthreads();
-  for (int i = tid; i < TH * TW; i += kAdjThreads) {
-    const int r = i / TW, cc = i % TW;
-    const int gy = y0 + r, gx = x0 + cc;
-    float b = 0.f, dz = 0.f;
-#pragma unroll
-    for (int dr = 0; dr < 3; ++dr)
-#pragma unroll
-      for (int dc = 0; dc < 3; ++dc) {
-        b += sQ[r + dr][cc + dc];
-        dz += sD[r + dr][cc + dc];
-      }
-    const float a = sA[r + 2][cc + 2];
-    const float ga = sG[r + 2][cc + 2] * b + dz;
-    if (gy < N && gx < N) gaff[(size_t)gy * N + gx] = ga * temp * a;
+  for (int j = 0; j < TH + 2; ++j) {
+    const int yq = y0 - 1 + j;
+    const size_t rn = (size_t)((yq + 1) & mask) * N;
+    const float a2 = A[rn], g2 = g[rn];
+    const float sv = s[(size_t)(yq & mask) * N];
+    const float hz2 = hsum3f(a2), ht2 = hsum3f(a2 * g2);
+    const float z = hz0 + hz1 + hz2, t1 = ht0 + ht1 + ht2;
+    const float q = sv / z;
+    const float dz = -t1 * q / z;
+    if (store && j >= 1 && j <= TH && yq < N) dsd[(size_t)yq * N] = t1 / z;
+    const float hq2 = hsum3f(q), hd2 = hsum3f(dz);
+    if (j >= 2) {
+      const int yo = yq - 1;
+      const float ga = g0 * (hq0 + hq1 + hq2) + (hd0 + hd1 + hd2);
+      if (store && yo < N) gaff[(size_t)yo * N] = ga * temp * a0;
+    }
+    hz0 = hz1; hz1 = hz2;
+    ht0 = ht1; ht1 = ht2;
+    hq0 = hq1; hq1 = hq2;
+    hd0 = hd1; hd1 = hd2;
+    a0 = a1; a1 = a2;
+    g0 = g1; g1 = g2;
   }
 }
 
@@ -370,7 +371,9 @@ static hipError_t launch_adjoint_n(const StepLaunch& sl, int gin, int gout, hipS
   hipError_t e = launch_step(N, fw, st);
   if (e != hipSuccess) return e;
   const int tiles_x = (N + kAdjTW - 1) / kAdjTW, tiles_y = (N + kAdjTH - 1) / kAdjTH;
-  k_diffusion_adj<<<dim3(tiles_x, tiles_y * sl.max_c, sl.count), kAdjThreads, 0, st>>>(sl.descs, sl.sin, gin, N);
+  const int waves = kAdjThreads / 64;
+  k_diffusion_adj<<<dim3((tiles_x * tiles_y + waves - 1) / waves, sl.max_c, sl.count), kAdjThreads, 0, st>>>(
+      sl.descs, sl.sin, gin, N);
   auto rows = [&](bool growth) {
     const int re = growth ? (sl.fwd_e ? sl.fwd_e : DefaultGeom<N>::FWD_E) : (sl.inv_e ? sl.inv_e : DefaultGeom<N>::INV_E);
     switch (re) {
