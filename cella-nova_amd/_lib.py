@@ -9,6 +9,8 @@ from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_size_t, c_uint
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcellanova.so")
+# An alternative build of the library can be loaded by path (e.g. a profiling build).
+LIB_PATH = os.environ.get("CELLANOVA_LIB", LIB_PATH)
 
 CN_OK = 0
 STATUS_NAMES = {

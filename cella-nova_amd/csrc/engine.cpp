@@ -54,13 +54,25 @@ bool hip_failed(hipError_t err, const char* what, cn_status* st) {
 // ---------------------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------------------
+// Twiddle table (layout in types.h): the kTwN base entries computed in FP64 and rounded once,
+// then the per-pass blocks as copies of base entries.
 std::vector<float> make_twiddles() {
-  std::vector<float> t(2 * kTwN);
+  std::vector<float> t(2 * (size_t)tw_table_size());
   for (int i = 0; i < kTwN; ++i) {
     const double a = -2.0 * M_PI * (double)i / (double)kTwN;
     t[2 * i] = (float)std::cos(a);
     t[2 * i + 1] = (float)std::sin(a);
   }
+  for (int r = 2; r <= 16; r *= 2)
+    for (int ns = 2; ns * r <= kTwN; ns *= 2) {
+      const int off = tw_pass_offset(ns, r), stride = kTwN / (ns * r);
+      for (int m = 1; m < r; ++m)
+        for (int k = 0; k < ns; ++k) {
+          const int src = (m * k * stride) % kTwN, dst = off + (m - 1) * ns + k;
+          t[2 * dst] = t[2 * src];
+          t[2 * dst + 1] = t[2 * src + 1];
+        }
+    }
   return t;
 }
 
@@ -587,7 +599,7 @@ cn_status create_engine(const cn_engine_desc* descs, uint32_t count, int device,
   for (uint32_t i = 0; i < count; ++i)
     if ((st = upload_desc(e, (int)i)) != CN_OK) return fail(st);
   void* p = nullptr;
-  if ((st = dalloc(e, e->allocs, &p, (size_t)kTwN * 8)) != CN_OK) return fail(st);
+  if ((st = dalloc(e, e->allocs, &p, (size_t)tw_table_size() * 8)) != CN_OK) return fail(st);
   e->tw = (float2*)p;
   const std::vector<float> tw = make_twiddles();
   if ((st = dalloc(e, e->allocs, &p, (size_t)count * kMaxChannels * e->mass_nblk * sizeof(double2))) != CN_OK)

@@ -21,9 +21,11 @@
 // "forward FFT -> pointwise product -> inverse FFT" keep the spectrum in registers.
 //
 // Twiddles come from a table tw[i] = exp(-2*pi*i*i/TWN) computed in FP64 on the host and
-// rounded once to FP32; an L-point transform uses stride TWN/L into it.
+// rounded once to FP32 (an L-point transform uses stride TWN/L into it), followed by per-pass
+// copies of the same values laid out for coalesced reads (types.h, tw_pass_offset).
 #pragma once
 #include <hip/hip_runtime.h>
+#include "types.h"
 
 namespace cn {
 
@@ -198,10 +200,13 @@ __device__ __forceinline__ void pass_compute(float2* v, int t, const float2* __r
     const int j = t + b * T;
     if constexpr (NS > 1) {
       const int k = j & (NS - 1);
-      constexpr int step = (TWN / L) * (L / (NS * R));  // table stride for W_{NS*R}
+      // Per-pass block of W_{NS*R}^{m*k}, k fastest: a wavefront's lanes read adjacent entries.
+      constexpr int off = tw_pass_offset(NS, R);
+      static_assert(off >= 0 && TWN == kTwN, "pass shape outside the twiddle table");
+      const float2* twp = tw + off + k;
 #pragma unroll
       for (int m = 1; m < R; ++m) {
-        const float2 w = tw[m * k * step];
+        const float2 w = twp[(m - 1) * NS];
         v[b * R + m] = INV ? c_mulc(v[b * R + m], w) : c_mul(v[b * R + m], w);
       }
     }

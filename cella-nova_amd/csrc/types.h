@@ -34,6 +34,27 @@ constexpr int kMaxChannels = 16;   // reference growth shader limit: array<f32,1
 constexpr int kMaxKernels = 256;
 constexpr int kTwN = 4096;         // twiddle table length; supports N <= 4096
 
+// Twiddle table layout: kTwN entries tw[i] = exp(-2 pi i i / kTwN), then one block per Stockham
+// pass shape (span NS, radix R), NS = 2 .. kTwN/R, R = 2, 4, 8, 16: entry (m - 1) * NS + k of
+// the block is W_{NS R}^{m k}, m = 1..R-1, k = 0..NS-1 (the same FP64-rounded value as
+// tw[m k kTwN / (NS R)]).  With k fastest, the lanes of a wavefront (consecutive k) read
+// consecutive entries.
+constexpr int tw_pass_offset(int NS, int R) {
+  int off = kTwN;
+  for (int r = 2; r <= 16; r *= 2)
+    for (int ns = 2; ns * r <= kTwN; ns *= 2) {
+      if (r == R && ns == NS) return off;
+      off += (r - 1) * ns;
+    }
+  return -1;
+}
+constexpr int tw_table_size() {
+  int n = kTwN;
+  for (int r = 2; r <= 16; r *= 2)
+    for (int ns = 2; ns * r <= kTwN; ns *= 2) n += (r - 1) * ns;
+  return n;
+}
+
 struct CreatureDev {
   float* state;
   int nslots;
