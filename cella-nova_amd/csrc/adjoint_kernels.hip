@@ -24,7 +24,7 @@
 
 namespace cn {
 
-constexpr int kAdjTW = 60, kAdjTH = 16, kAdjThreads = 256;  // output columns / rows per wave
+constexpr int kAdjTW = 60, kAdjTH = 8, kAdjThreads = 256;   // output columns / rows per wave
 
 // x(l-1) + x(l) + x(l+1) across the wavefront (DPP wave shifts; lanes 0 and 63 see a 0).
 __device__ __forceinline__ float hsum3f(float x) {
@@ -65,19 +65,26 @@ k_diffusion_adj(const CreatureDev* __restrict__ descs, int sin, int gin, int N) 
   float* gaff = d.gaff + c * plane + xl;
   const float temp = d.temp;
 
-  // Step j handles q row yq = y0 - 1 + j; from j = 2 on it finishes gaff of row yq - 1.
-  const size_t rm2 = (size_t)((y0 - 2) & mask) * N, rm1 = (size_t)((y0 - 1) & mask) * N;
-  const float am2 = A[rm2], am1 = A[rm1];
-  float hz0 = hsum3f(am2), hz1 = hsum3f(am1);              // horizontal sums of A
-  float ht0 = hsum3f(am2 * g[rm2]), ht1 = hsum3f(am1 * g[rm1]);  // ... of A g
+  // All rows of the tile are loaded up front; step j handles q row yq = y0 - 1 + j and from
+  // j = 2 on finishes gaff of row yq - 1.
+  float ar[TH + 4], gr[TH + 4], sr[TH + 2];
+#pragma unroll
+  for (int j = 0; j < TH + 4; ++j) {
+    const size_t o = (size_t)((y0 - 2 + j) & mask) * N;
+    ar[j] = A[o];
+    gr[j] = g[o];
+  }
+#pragma unroll
+  for (int j = 0; j < TH + 2; ++j) sr[j] = s[(size_t)((y0 - 1 + j) & mask) * N];
+  float hz0 = hsum3f(ar[0]), hz1 = hsum3f(ar[1]);                  // horizontal sums of A
+  float ht0 = hsum3f(ar[0] * gr[0]), ht1 = hsum3f(ar[1] * gr[1]);  // ... of A g
   float hq0 = 0.f, hq1 = 0.f, hd0 = 0.f, hd1 = 0.f;        // ... of q and dZ (previous q rows)
   float a0 = 0.f, a1 = 0.f, g0 = 0.f, g1 = 0.f;            // A, g of the previous step rows
 #pragma unroll
   for (int j = 0; j < TH + 2; ++j) {
     const int yq = y0 - 1 + j;
-    const size_t rn = (size_t)((yq + 1) & mask) * N;
-    const float a2 = A[rn], g2 = g[rn];
-    const float sv = s[(size_t)(yq & mask) * N];
+    const float a2 = ar[j + 2], g2 = gr[j + 2];
+    const float sv = sr[j];
     const float hz2 = hsum3f(a2), ht2 = hsum3f(a2 * g2);
     const float z = hz0 + hz1 + hz2, t1 = ht0 + ht1 + ht2;
     const float q = sv / z;

@@ -307,17 +307,23 @@ k_diffusion(const CreatureDev* __restrict__ descs, int sin, int sout, int N, int
   // byte of the packed 0x00RRGGBB pixel (little endian: R is byte 2, B byte 0).
   unsigned char* rgb = (render && c < 3 && d.render) ? reinterpret_cast<unsigned char*>(d.render) : nullptr;
 
-  // Step j computes q on row y0 - 1 + j; from j = 2 on it also finishes output row y0 + j - 2.
-  double h0 = hsum3(A[(size_t)((y0 - 2) & mask) * N]);  // horizontal A sums of rows yq-1, yq
-  double h1 = hsum3(A[(size_t)((y0 - 1) & mask) * N]);
+  // All rows of the tile are loaded up front (independent loads in flight together), then
+  // step j computes q on row y0 - 1 + j; from j = 2 on it also finishes output row y0 + j - 2.
+  float ar[TH + 4], sr[TH + 2];
+#pragma unroll
+  for (int j = 0; j < TH + 4; ++j) ar[j] = A[(size_t)((y0 - 2 + j) & mask) * N];
+#pragma unroll
+  for (int j = 0; j < TH + 2; ++j) sr[j] = s[(size_t)((y0 - 1 + j) & mask) * N];
+  double h0 = hsum3(ar[0]);  // horizontal A sums of rows yq-1, yq
+  double h1 = hsum3(ar[1]);
   double g0 = 0.0, g1 = 0.0;  // horizontal q sums of the two previous q rows
   float a0 = 0.f, a1 = 0.f;   // A of the two previous A rows loaded in the steps
   float r1 = 0.f;             // residual of the previous q row
 #pragma unroll
   for (int j = 0; j < TH + 2; ++j) {
     const int yq = y0 - 1 + j;
-    const float a2 = A[(size_t)((yq + 1) & mask) * N];
-    const float sv = s[(size_t)(yq & mask) * N];
+    const float a2 = ar[j + 2];
+    const float sv = sr[j];
     const double h2 = hsum3(a2);
     const double z = h0 + h1 + h2;
     const float q = sv * __builtin_amdgcn_rcpf((float)z);
