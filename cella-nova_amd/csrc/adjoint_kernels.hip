@@ -174,6 +174,11 @@ k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) 
   const int x = x0 + lane;
   const bool edge = (x0 == 0);
   __shared__ float2 lds[LdsMap<L, TC, true>::size];
+  using TF = TwLds<L, E, false>;
+  using TI = TwLds<L, E, true>;
+  __shared__ float2 twl[TF::size + TI::size + 1];  // both plans' twiddles (see TwLds)
+  TF::fill(twl, tw, threadIdx.x, blockDim.x);
+  TI::fill(twl + TF::size, tw, threadIdx.x, blockDim.x);
 
   // acc += v * conj(K^_k) at this thread's positions (compact real copy when available)
   auto accumulate = [&](int k, const float2* v, float2* acc) {
@@ -205,7 +210,7 @@ k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) 
     float2 v[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) v[i] = sp[(size_t)in_pos<L, E, false>(t, i) * HALF + x];
-    fft_run<L, E, TC, true, false, kTwN>(v, lds, t, lane, tw);
+    fft_run<L, E, TC, true, false, kTwN, 0, true>(v, lds, t, lane, twl);
     if (edge) {
       // Column 0 packs (DC column) + i (Nyquist column): split, multiply each by its
       // conjugate kernel column, re-pack.
@@ -236,7 +241,7 @@ k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) 
       accumulate(k, v, acc);
     }
   }
-  fft_run<L, E, TC, true, true, kTwN>(acc, lds, t, lane, tw);
+  fft_run<L, E, TC, true, true, kTwN, 0, true>(acc, lds, t, lane, twl + TF::size);
   float2* out = reinterpret_cast<float2*>(d.spec) + (size_t)c * N * HALF;
 #pragma unroll
   for (int i = 0; i < E; ++i) out[(size_t)out_pos<L, E, true>(t, i) * HALF + x] = acc[i];

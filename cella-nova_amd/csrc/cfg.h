@@ -33,10 +33,12 @@ struct DefaultGeom {
 };
 
 // Column geometries a launcher may pick for size N: at most 1024 threads per block and the
-// padded exchange buffer (N + N/16) * TC * 8 bytes within 160 KiB of LDS.
+// padded exchange buffer (N + N/16) * TC * 8 bytes within 152 KiB of LDS (the rest of the
+// 160 KiB holds the column kernels' twiddle tables, at most 8 KiB for N <= 2048).
+constexpr long kColLdsBudget = 152 * 1024;
 template <int N, int TC, int EC>
 constexpr bool col_tc_ok() {
-  return TC <= N / 2 && TC * ColCfg<N, EC>::T <= 1024 && (long)(N + N / 16) * TC * 8 <= 160 * 1024;
+  return TC <= N / 2 && TC * ColCfg<N, EC>::T <= 1024 && (long)(N + N / 16) * TC * 8 <= kColLdsBudget;
 }
 
 // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs (separate L2s); give

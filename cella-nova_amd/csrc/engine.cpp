@@ -14,6 +14,7 @@
 #include <vector>
 #include "common.h"
 #include "engine.h"
+#include "cfg.h"
 #include "launch.h"
 
 namespace cn {
@@ -1196,7 +1197,7 @@ namespace {
 
 bool col_geom_ok(int N, int ce, int tc) {
   const int E = std::min(ce, N), T = N / E;
-  return tc <= N / 2 && tc * T <= 1024 && (long)(N + N / 16) * tc * 8 <= 160L * 1024;
+  return tc <= N / 2 && tc * T <= 1024 && (long)(N + N / 16) * tc * 8 <= kColLdsBudget;  // cfg.h
 }
 
 struct Geom {

@@ -190,8 +190,55 @@ struct LdsMap {
   static constexpr int off(int c) { return INTERLEAVED ? (c + c / 16) * NT : c + c / 16; }
 };
 
-// One Stockham pass on register data (twiddle + radix-R butterflies) for thread t.
-template <int L, int E, int R, int NS, bool INV, int TWN>
+// LDS copy of the twiddles of one plan (L, E, INV), for transforms whose global twiddle loads
+// would share the in-order vector-memory counter with the kernel's own loads and stores (a wait
+// for a twiddle would also wait for every older prefetch and store).  For each pass p >= 1
+// (span NS, radix R, M = NS R) either the direct block D[m-1][k] = W_M^{m k} when
+// (R-1) NS <= 256, or two factors with k = 16 k_hi + k_lo: B[m-1][k_lo] = W_M^{m k_lo} followed
+// by A[m-1][k_hi] = W_M^{16 m k_hi}, so that W_M^{m k} = A B (one complex product, ~2 ulp).
+template <int L, int E, bool INV>
+struct TwLds {
+  using PL = Plan<L, E>;
+  static constexpr int R(int p) { return PL::radix(p, INV); }
+  static constexpr int NS(int p) { return PL::span(p, INV); }
+  static constexpr bool direct(int p) { return (R(p) - 1) * NS(p) <= 256; }
+  static constexpr int pass_size(int p) {
+    return direct(p) ? (R(p) - 1) * NS(p) : (R(p) - 1) * (16 + NS(p) / 16);
+  }
+  static constexpr int off(int p) {
+    int o = 0;
+    for (int q = 1; q < p; ++q) o += pass_size(q);
+    return o;
+  }
+  static constexpr int size = off(PL::npass);
+
+  // Block-cooperative fill from the global table (layout in types.h); the caller's first
+  // exchange barrier orders it before any use.
+  template <int P = 1>
+  static __device__ __forceinline__ void fill(float2* dst, const float2* __restrict__ tw, int tid, int nth) {
+    if constexpr (P < PL::npass) {
+      constexpr int r = R(P), ns = NS(P), m_n = r * ns;
+      const float2* tp = tw + tw_pass_offset(ns, r);
+      float2* d = dst + off(P);
+      if constexpr (direct(P)) {
+        for (int i = tid; i < (r - 1) * ns; i += nth) d[i] = tp[i];
+      } else {
+        for (int i = tid; i < (r - 1) * 16; i += nth) d[i] = tp[(i / 16) * ns + (i % 16)];
+        constexpr int nh = ns / 16;
+        for (int i = tid; i < (r - 1) * nh; i += nth) {
+          const int m = i / nh + 1, kh = i % nh;
+          d[(r - 1) * 16 + i] = tw[((16 * m * kh) * (kTwN / m_n)) % kTwN];
+        }
+      }
+      fill<P + 1>(dst, tw, tid, nth);
+    }
+  }
+};
+
+// One Stockham pass on register data (twiddle + radix-R butterflies) for thread t.  Pass P of
+// the plan; LT: the twiddles come from TwLds<L, E, INV> at `tw` (LDS), else from the global
+// table.
+template <int L, int E, int R, int NS, bool INV, int TWN, int P, bool LT>
 __device__ __forceinline__ void pass_compute(float2* v, int t, const float2* __restrict__ tw) {
   constexpr int T = L / E;
   constexpr int NB = E / R;
@@ -200,14 +247,35 @@ __device__ __forceinline__ void pass_compute(float2* v, int t, const float2* __r
     const int j = t + b * T;
     if constexpr (NS > 1) {
       const int k = j & (NS - 1);
-      // Per-pass block of W_{NS*R}^{m*k}, k fastest: a wavefront's lanes read adjacent entries.
-      constexpr int off = tw_pass_offset(NS, R);
-      static_assert(off >= 0 && TWN == kTwN, "pass shape outside the twiddle table");
-      const float2* twp = tw + off + k;
+      if constexpr (LT) {
+        using TL = TwLds<L, E, INV>;
+        static_assert(TL::R(P) == R && TL::NS(P) == NS, "LDS twiddle layout / plan mismatch");
+        if constexpr (TL::direct(P)) {
+          const float2* twp = tw + TL::off(P) + k;
 #pragma unroll
-      for (int m = 1; m < R; ++m) {
-        const float2 w = twp[(m - 1) * NS];
-        v[b * R + m] = INV ? c_mulc(v[b * R + m], w) : c_mul(v[b * R + m], w);
+          for (int m = 1; m < R; ++m) {
+            const float2 w = twp[(m - 1) * NS];
+            v[b * R + m] = INV ? c_mulc(v[b * R + m], w) : c_mul(v[b * R + m], w);
+          }
+        } else {
+          const float2* tb = tw + TL::off(P) + (k & 15);
+          const float2* ta = tw + TL::off(P) + (R - 1) * 16 + (k >> 4);
+#pragma unroll
+          for (int m = 1; m < R; ++m) {
+            const float2 w = c_mul(ta[(m - 1) * (NS / 16)], tb[(m - 1) * 16]);
+            v[b * R + m] = INV ? c_mulc(v[b * R + m], w) : c_mul(v[b * R + m], w);
+          }
+        }
+      } else {
+        // Per-pass block of W_{NS*R}^{m*k}, k fastest: a wavefront's lanes read adjacent entries.
+        constexpr int off = tw_pass_offset(NS, R);
+        static_assert(off >= 0 && TWN == kTwN, "pass shape outside the twiddle table");
+        const float2* twp = tw + off + k;
+#pragma unroll
+        for (int m = 1; m < R; ++m) {
+          const float2 w = twp[(m - 1) * NS];
+          v[b * R + m] = INV ? c_mulc(v[b * R + m], w) : c_mul(v[b * R + m], w);
+        }
       }
     }
     Dft<R, INV>::run(v + b * R);
@@ -242,19 +310,20 @@ __device__ __forceinline__ int launder(int x) {
   return x;
 }
 
-// Runs passes P..npass-1 of an L-point transform.  On entry v holds the inputs of pass P
-// (layout of pass P); on exit it holds the outputs of the last pass.  `lds` must hold
+// Runs passes P..npass-1 of an L-point transform (LT: twiddles from an LDS TwLds block).  On
+// entry v holds the inputs of pass P (layout of pass P); on exit it holds the outputs of the last
+// pass.  `lds` must hold
 // LdsMap<L,NT,IL>::size elements; every exchange is bracketed by barriers (wavefront-level for
 // row-layout transforms that fit in a wavefront, block-level otherwise), so the routine can be
 // called repeatedly on the same buffer.
-template <int L, int E, int NT, bool IL, bool INV, int TWN, int P = 0>
+template <int L, int E, int NT, bool IL, bool INV, int TWN, int P = 0, bool LT = false>
 __device__ __forceinline__ void fft_run(float2* v, float2* lds, int t, int lane,
                                         const float2* __restrict__ tw) {
   using PL = Plan<L, E>;
   constexpr int R = PL::radix(P, INV);
   constexpr int NS = PL::span(P, INV);
   t = launder(t);
-  pass_compute<L, E, R, NS, INV, TWN>(v, t, tw);
+  pass_compute<L, E, R, NS, INV, TWN, P, LT>(v, t, tw);
   if constexpr (P + 1 < PL::npass) {
     constexpr int T = L / E;
     constexpr int R2 = PL::radix(P + 1, INV);
@@ -293,7 +362,7 @@ __device__ __forceinline__ void fft_run(float2* v, float2* lds, int t, int lane,
         for (int m = 0; m < R2; ++m) v[b * R2 + m] = lds[LM::at(j + m * (L / R2), lane)];
       }
     }
-    fft_run<L, E, NT, IL, INV, TWN, P + 1>(v, lds, t, lane, tw);
+    fft_run<L, E, NT, IL, INV, TWN, P + 1, LT>(v, lds, t, lane, tw);
   }
 }
 

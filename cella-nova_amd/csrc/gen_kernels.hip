@@ -88,7 +88,10 @@ k_gen_cols(const CreatureDev* __restrict__ desc, const float2* __restrict__ tw) 
 #pragma unroll
   for (int i = 0; i < E; ++i) v[i] = src[(size_t)in_pos<L, E, false>(t, i) * HALF + x];
   __shared__ float2 lds[LdsMap<L, TC, true>::size];
-  fft_run<L, E, TC, true, false, kTwN>(v, lds, t, lane, tw);
+  using TF = TwLds<L, E, false>;
+  __shared__ float2 twl[TF::size + 1];  // forward-plan twiddles (see TwLds)
+  TF::fill(twl, tw, threadIdx.x, blockDim.x);
+  fft_run<L, E, TC, true, false, kTwN, 0, true>(v, lds, t, lane, twl);
 
   const float sc = 1.0f / ((float)N * (float)N);  // exact: N is a power of two
   float2* kh = const_cast<float2*>(reinterpret_cast<const float2*>(d.khalf)) + (size_t)k * N * HALF;

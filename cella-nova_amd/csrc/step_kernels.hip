@@ -95,7 +95,13 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
   float2 nk[E];
   load_kernel(d.src_k[kb], nk);
   __shared__ float2 lds[LdsMap<L, TC, true>::size];
-  fft_run<L, E, TC, true, false, kTwN>(v, lds, t, lane, tw);
+  // Twiddles of both plans in LDS (see TwLds); the first exchange barrier orders the fill.
+  using TF = TwLds<L, E, false>;
+  using TI = TwLds<L, E, true>;
+  __shared__ float2 twl[TF::size + TI::size + 1];
+  TF::fill(twl, tw, threadIdx.x, blockDim.x);
+  TI::fill(twl + TF::size, tw, threadIdx.x, blockDim.x);
+  fft_run<L, E, TC, true, false, kTwN, 0, true>(v, lds, t, lane, twl);
 
   // Tile 0, column 0 carries two real columns packed as (DC column) + i (Nyquist column).
   // Per kernel, lane 0 splits its spectrum F into A = FFT(DC column) and B = FFT(Nyquist
@@ -147,7 +153,7 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
         }
       }
     }
-    fft_run<L, E, TC, true, true, kTwN>(w, lds, t, lane, tw);
+    fft_run<L, E, TC, true, true, kTwN, 0, true>(w, lds, t, lane, twl + TF::size);
     float2* uo = reinterpret_cast<float2*>(d.uhat) + (size_t)k * N * HALF;
 #pragma unroll
     for (int i = 0; i < E; ++i) uo[(size_t)out_pos<L, E, true>(t, i) * HALF + x] = w[i];
