@@ -8,10 +8,28 @@
 
 namespace cn {
 
+// W_N^p, 0 <= p < N/2, for the real-row split/merge from a small LDS table pair:
+// p = 32 p_hi + p_lo, lo[p_lo] = W_N^{p_lo}, hi[p_hi] = W_N^{32 p_hi}, W_N^p = hi * lo.
+template <int N>
+struct SplitTw {
+  static constexpr int NLO = 32, NHI = (N / 2 + 31) / 32;
+  static constexpr int size = NLO + NHI;
+  static __device__ __forceinline__ void fill(float2* d, const float2* __restrict__ tw, int tid, int nth) {
+    for (int i = tid; i < size; i += nth) {
+      const int e = i < NLO ? i : 32 * (i - NLO);
+      d[i] = tw[(e * (kTwN / N)) % kTwN];
+    }
+  }
+  static __device__ __forceinline__ float2 get(const float2* d, int p) {
+    return c_mul(d[NLO + (p >> 5)], d[p & 31]);
+  }
+};
+
 // After the forward L-point transform of z (L = N/2), v holds Z at out_pos positions; turn it
 // into the real row's half spectrum and store it to `out` (L complex).  Uses `lds` (the same
 // buffer as the transform) for the Z[L-k] partner exchange.
-template <int N, int E, int ROWS>
+// LT: `tw` is a SplitTw<N> table in LDS, else the global table.
+template <int N, int E, int ROWS, bool LT = false>
 __device__ __forceinline__ void r2c_finish(const float2* v, float2* lds, int t, int lane,
                                            const float2* __restrict__ tw, float2* out, bool store) {
   constexpr int L = N / 2;
@@ -35,7 +53,7 @@ __device__ __forceinline__ void r2c_finish(const float2* v, float2* lds, int t, 
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const int p = out_pos<L, E, false>(t, i);
-    w[i] = tw[p * (kTwN / N)];
+    w[i] = LT ? SplitTw<N>::get(tw, p) : tw[p * (kTwN / N)];
     zq[i] = lds[LdsMap<L, ROWS, false>::at((L - p) & (L - 1), lane)];
   }
 #pragma unroll
@@ -67,7 +85,7 @@ __device__ __forceinline__ void c2r_load(const float2* __restrict__ X, int t, fl
   }
 }
 
-template <int N, int E>
+template <int N, int E, bool LT = false>
 __device__ __forceinline__ void c2r_form(const float2* xp, const float2* xq, int t, const float2* __restrict__ tw,
                                          float2* v) {
   constexpr int L = N / 2;
@@ -77,7 +95,7 @@ __device__ __forceinline__ void c2r_form(const float2* xp, const float2* xq, int
     const int p = in_pos<L, E, true>(t, i);
     const float2 e2 = make_float2(xp[i].x + xq[i].x, xp[i].y - xq[i].y);  // xp + conj(xq)
     const float2 dd = make_float2(xp[i].x - xq[i].x, xp[i].y + xq[i].y);  // xp - conj(xq)
-    const float2 o2 = c_mulc(dd, tw[p * (kTwN / N)]);                      // * W_N^{-p}
+    const float2 o2 = c_mulc(dd, LT ? SplitTw<N>::get(tw, p) : tw[p * (kTwN / N)]);  // * W_N^{-p}
     const float2 g = make_float2(e2.x - o2.y, e2.y + o2.x);               // e2 + i*o2
     const float2 g0 = make_float2(xp[i].x + xp[i].y, xp[i].x - xp[i].y);
     v[i] = (p == 0) ? g0 : g;

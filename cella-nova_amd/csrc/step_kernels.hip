@@ -42,11 +42,19 @@ k_row_fwd(const CreatureDev* __restrict__ descs, int sin, const float2* __restri
 #pragma unroll
   for (int i = 0; i < E; ++i) v[i] = in[in_pos<L, E, false>(t, i)];
 
+  // Twiddles in LDS (the row's own loads and stores do not wait behind them); one-wavefront
+  // rows exchange at wavefront level, so the fill gets a block barrier of its own.
+  using TF = TwLds<L, E, false>;
+  __shared__ float2 twl[TF::size + SplitTw<N>::size];
+  TF::fill(twl, tw, threadIdx.x, blockDim.x);
+  SplitTw<N>::fill(twl + TF::size, tw, threadIdx.x, blockDim.x);
+  __syncthreads();
   __shared__ float2 lds[LdsMap<L, ROWS, false>::size];
-  fft_run<L, E, ROWS, false, false, kTwN>(v, lds, t, lane, tw);
+  fft_run<L, E, ROWS, false, false, kTwN, 0, true>(v, lds, t, lane, twl);
 
   // Split the packed transform Z of z[n] = x[2n] + i x[2n+1] into the real row's spectrum.
-  r2c_finish<N, E, ROWS>(v, lds, t, lane, tw, reinterpret_cast<float2*>(d.spec) + (size_t)row * L, valid);
+  r2c_finish<N, E, ROWS, true>(v, lds, t, lane, twl + TF::size,
+                               reinterpret_cast<float2*>(d.spec) + (size_t)row * L, valid);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -175,6 +183,11 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
   const int lane = threadIdx.x / T, t0 = threadIdx.x % T;
   const int y = blockIdx.x * ROWS + lane;  // ROWS divides N
   __shared__ float2 lds[LdsMap<L, ROWS, false>::size];
+  using TI = TwLds<L, E, true>;
+  __shared__ float2 twl[TI::size + SplitTw<N>::size];  // LDS twiddles, as in k_row_fwd
+  TI::fill(twl, tw, threadIdx.x, blockDim.x);
+  SplitTw<N>::fill(twl + TI::size, tw, threadIdx.x, blockDim.x);
+  __syncthreads();
   const float temp = d.temp;
   {
     float2 acc[E];
@@ -202,14 +215,14 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
       float2 v[E];
       if constexpr (!kPrefetch) c2r_load_own<N, E>(uhat + ((size_t)k * N + y) * L, t, xp);
       if constexpr (kWave) c2r_partner<N, E, ROWS>(xp, lds, t, lane, xq);
-      c2r_form<N, E>(xp, xq, t, tw, v);
+      c2r_form<N, E, true>(xp, xq, t, twl + TI::size, v);
       if (kPrefetch && kk + 1 < ke) {
         if constexpr (kWave)
           c2r_load_own<N, E>(uhat + ((size_t)d.dst_k[kk + 1] * N + y) * L, t, xp);
         else
           c2r_load<N, E>(uhat + ((size_t)d.dst_k[kk + 1] * N + y) * L, t, xp, xq);
       }
-      fft_run<L, E, ROWS, false, true, kTwN>(v, lds, t, lane, tw);
+      fft_run<L, E, ROWS, false, true, kTwN, 0, true>(v, lds, t, lane, twl);
 #pragma unroll
       for (int i = 0; i < E; ++i) {
         const int n = out_pos<L, E, true>(t, i);
