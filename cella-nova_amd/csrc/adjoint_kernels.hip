@@ -140,10 +140,15 @@ k_growth_adj_row(const CreatureDev* __restrict__ descs, const float2* __restrict
   }
   __shared__ float2 lds[LdsMap<L, ROWS, false>::size];
   __shared__ float red[3][NT];
+  using TF = TwLds<L, E, false>;
+  __shared__ float2 twl[TF::size + SplitTw<N>::size];  // LDS twiddles (see k_row_fwd)
+  TF::fill(twl, tw, threadIdx.x, blockDim.x);
+  SplitTw<N>::fill(twl + TF::size, tw, threadIdx.x, blockDim.x);
   red[0][threadIdx.x] = smu;
   red[1][threadIdx.x] = ssg;
   red[2][threadIdx.x] = sw;
-  fft_run<L, E, ROWS, false, false, kTwN>(v, lds, t, lane, tw);
+  __syncthreads();
+  fft_run<L, E, ROWS, false, false, kTwN, 0, true>(v, lds, t, lane, twl);
   __syncthreads();  // orders the red[] writes above before the reduction
   for (int h = NT / 2; h > 0; h >>= 1) {
     if (threadIdx.x < h)
@@ -152,7 +157,8 @@ k_growth_adj_row(const CreatureDev* __restrict__ descs, const float2* __restrict
     __syncthreads();
   }
   if (threadIdx.x < 3) d.partial[((size_t)blockIdx.x * d.K + k) * 3 + threadIdx.x] = red[threadIdx.x][0];
-  r2c_finish<N, E, ROWS>(v, lds, t, lane, tw, reinterpret_cast<float2*>(d.specg) + ((size_t)k * N + y) * L, true);
+  r2c_finish<N, E, ROWS, true>(v, lds, t, lane, twl + TF::size,
+                               reinterpret_cast<float2*>(d.specg) + ((size_t)k * N + y) * L, true);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -262,9 +268,14 @@ k_row_inv_add(const CreatureDev* __restrict__ descs, int gout, const float2* __r
   const int lane = threadIdx.x / T, t = threadIdx.x % T;
   const int y = blockIdx.x * ROWS + lane;
   __shared__ float2 lds[LdsMap<L, ROWS, false>::size];
+  using TI = TwLds<L, E, true>;
+  __shared__ float2 twl[TI::size + SplitTw<N>::size];  // LDS twiddles (see k_row_fwd)
+  TI::fill(twl, tw, threadIdx.x, blockDim.x);
+  SplitTw<N>::fill(twl + TI::size, tw, threadIdx.x, blockDim.x);
+  __syncthreads();
   float2 v[E];
-  c2r_start<N, E>(reinterpret_cast<const float2*>(d.spec) + ((size_t)c * N + y) * L, t, tw, v);
-  fft_run<L, E, ROWS, false, true, kTwN>(v, lds, t, lane, tw);
+  c2r_start<N, E, true>(reinterpret_cast<const float2*>(d.spec) + ((size_t)c * N + y) * L, t, twl + TI::size, v);
+  fft_run<L, E, ROWS, false, true, kTwN, 0, true>(v, lds, t, lane, twl);
   const size_t row = ((size_t)c * N + y) * N;
   const float2* dd = reinterpret_cast<const float2*>(d.dsd + row);
   float2* o = reinterpret_cast<float2*>(d.grad + (size_t)gout * d.C * N * N + row);

@@ -69,8 +69,14 @@ k_gen_rows(const CreatureDev* __restrict__ desc, const RingParams* __restrict__ 
     v[i] = make_float2((float)a, (float)b);
   }
   __shared__ float2 lds[LdsMap<L, ROWS, false>::size];
-  fft_run<L, E, ROWS, false, false, kTwN>(v, lds, t, lane, tw);
-  r2c_finish<N, E, ROWS>(v, lds, t, lane, tw, reinterpret_cast<float2*>(d.uhat) + (size_t)row * L, valid);
+  using TF = TwLds<L, E, false>;
+  __shared__ float2 twl[TF::size + SplitTw<N>::size];  // LDS twiddles (see k_row_fwd)
+  TF::fill(twl, tw, threadIdx.x, blockDim.x);
+  SplitTw<N>::fill(twl + TF::size, tw, threadIdx.x, blockDim.x);
+  __syncthreads();
+  fft_run<L, E, ROWS, false, false, kTwN, 0, true>(v, lds, t, lane, twl);
+  r2c_finish<N, E, ROWS, true>(v, lds, t, lane, twl + TF::size, reinterpret_cast<float2*>(d.uhat) + (size_t)row * L,
+                               valid);
 }
 
 template <int N, int TC, int EC>

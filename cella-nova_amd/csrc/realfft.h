@@ -125,12 +125,12 @@ __device__ __forceinline__ void c2r_partner(const float2* xp, float2* lds, int t
   for (int i = 0; i < E; ++i) xq[i] = lds[LM::at((L - in_pos<L, E, true>(t, i)) & (L - 1), lane)];
 }
 
-template <int N, int E>
+template <int N, int E, bool LT = false>
 __device__ __forceinline__ void c2r_start(const float2* __restrict__ X, int t, const float2* __restrict__ tw,
                                           float2* v) {
   float2 xp[E], xq[E];
   c2r_load<N, E>(X, t, xp, xq);
-  c2r_form<N, E>(xp, xq, t, tw, v);
+  c2r_form<N, E, LT>(xp, xq, t, tw, v);
 }
 
 }  // namespace cn
