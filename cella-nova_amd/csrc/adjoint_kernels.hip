@@ -193,7 +193,7 @@ k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) 
 #pragma unroll
       for (int i = 0; i < E; ++i) {
         const int y = out_pos<L, E, false>(t, i);
-        const float r = kr[(size_t)(y <= HALF ? y : N - y) * HALF + x];
+        const float r = kr[kreal_index(N, y <= HALF ? y : N - y, x)];
         acc[i] = make_float2(acc[i].x + v[i].x * r, acc[i].y + v[i].y * r);
       }
     } else {

@@ -162,7 +162,8 @@ void project_kernel(const float* full, uint32_t N, float* half, float* nyq) {
 
 // True if the (Hermitian-projected) spectrum is real and even in ky to within f32 rounding
 // of its largest coefficient (|Im| and |K(ky) - K(-ky)| <= 2^-23 max|K|), as for the
-// radially symmetric Lenia kernels; then fills `real` with rows ky = 0 .. N/2 of Re K.
+// radially symmetric Lenia kernels; then fills `real` with rows ky = 0 .. N/2 of Re K in the
+// kreal layout (types.h).
 bool real_even_spectrum(const float* half, const float* nyq, uint32_t N, std::vector<float>& real) {
   const size_t H = N / 2;
   double mx = 0.0;
@@ -179,7 +180,7 @@ bool real_even_spectrum(const float* half, const float* nyq, uint32_t N, std::ve
   }
   real.resize((H + 1) * H);
   for (size_t y = 0; y <= H; ++y)
-    for (size_t x = 0; x < H; ++x) real[y * H + x] = half[2 * (y * H + x)];
+    for (size_t x = 0; x < H; ++x) real[kreal_index((int)N, (int)y, (int)x)] = half[2 * (y * H + x)];
   return true;
 }
 

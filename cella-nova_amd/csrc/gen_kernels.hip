@@ -129,7 +129,7 @@ k_gen_cols(const CreatureDev* __restrict__ desc, const float2* __restrict__ tw) 
     const int y = out_pos<L, E, false>(t, i);
     const float2 val = make_float2(v[i].x * sc, v[i].y * sc);
     kh[(size_t)y * HALF + x] = val;
-    if (y <= HALF) kr[(size_t)y * HALF + x] = val.x;
+    if (y <= HALF) kr[kreal_index(N, y, x)] = val.x;
   }
 }
 

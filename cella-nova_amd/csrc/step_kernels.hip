@@ -92,7 +92,7 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
 #pragma unroll
       for (int i = 0; i < E; ++i) {
         const int y = out_pos<L, E, false>(t, i);
-        out[i] = make_float2(kr[(size_t)(y <= HALF ? y : N - y) * HALF + x], 0.0f);
+        out[i] = make_float2(kr[kreal_index(N, y <= HALF ? y : N - y, x)], 0.0f);
       }
     } else {
       const float2* kh = khalf + (size_t)k * N * HALF;

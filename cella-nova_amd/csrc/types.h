@@ -8,8 +8,10 @@
 //   khalf   [K][N][N/2]  c64   Hermitian part of each kernel spectrum, pre-scaled by 1/N^2,
 //                              columns kx = 0 .. N/2-1
 //   knyq    [K][N]       c64   the kx = N/2 (Nyquist) column of the same spectra
-//   kreal   [K][N/2+1][N/2] f32 compact copy for kernels whose spectrum is real and even in ky
-//                              (rows ky = 0 .. N/2; row N-ky equals row ky); ksym[k] = 1
+//   kreal   [K][N/8][N/2+1][4] f32 compact copy for kernels whose spectrum is real and even in
+//                              ky (rows ky = 0 .. N/2; row N-ky equals row ky), in tiles of 4
+//                              columns (kreal_index) so that column tiles read contiguous runs;
+//                              ksym[k] = 1
 //   uhat    [K][N][N/2]  c64   row spectra of every convolution result (after the inverse
 //                              column pass); slot kx = 0 packs (DC, Nyquist) as (re, im)
 //   A       [C][N][N]    f32   affinity exp(temp * sum_k w_k G_k) per destination channel
@@ -90,6 +92,14 @@ struct CreatureDev {
   float w[kMaxKernels];
   int ksym[kMaxKernels];       // 1: kernel k is read from kreal (real, even in ky)
 };
+
+// Offset of (ky <= N/2, kx < N/2) in one kernel's kreal block: 4-column tiles, rows inside.
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+inline size_t kreal_index(int N, int ky, int kx) {
+  return ((size_t)(kx >> 2) * (N / 2 + 1) + ky) * 4 + (kx & 3);
+}
 
 #ifdef __HIPCC__
 __device__ __forceinline__ float* slot_ptr(const CreatureDev& d, int s) {
