@@ -296,7 +296,7 @@ def main():
         eng.set_plan(n, fe, ie, ce, tc, th)
         plan = eng.plans()[0]
     else:
-        plan = eng.plans()[0] if args.no_autotune else eng.autotune(5)[0]
+        plan = eng.plans()[0] if args.no_autotune else eng.autotune(9)[0]
     try:
         import torch
         sync_dev = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
