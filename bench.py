@@ -237,6 +237,38 @@ def batched_forward(pkg, steps=200, seed=0):
             "setup_s_device_generator": round(setup_dev, 3), "setup_s_host_generator_est": round(t_host, 1)}
 
 
+def rollout_grad(pkg, n=1024, R=13, T=64, ckpt=8, iters=5, torch_iters=1, seed=0):
+    """Differentiable rollout (config #4): one 1024^2 C3K9 creature, T steps with a checkpoint
+    every `ckpt`, MSE to the state a second seeded creature reaches after T steps, gradients for
+    the state and mu/sigma/w through the hand-written adjoint (one HIP graph per rollout).
+    Reported as grad-steps/s and ms per rollout, next to the reference's PyTorch autograd path
+    on the same creature (torch_reference_rate)."""
+    cr = pkg.baseline_creature(n, num_channels=3, radius=R, seed=seed)
+    tg = pkg.baseline_creature(n, num_channels=3, radius=R, seed=seed + 1)
+    te = pkg.make_engine(tg)
+    te.run_steps(T)
+    target = te.download_all_channels().reshape(3, n, n)
+    te.close()
+    eng = pkg.make_engine(cr)
+    eng.creature_set_target(0, target)
+    eng.rollout_grad(T, ckpt)  # captures the graph
+    eng.sync()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        eng.rollout_grad(T, ckpt)
+    eng.sync()
+    dt = (time.perf_counter() - t0) / iters
+    loss = eng.creature_loss(0)
+    eng.close()
+    out = {"config": f"{n}x{n} C3K9 R={R}, T={T}, checkpoint every {ckpt}, {iters} rollouts",
+           "ms_per_rollout": round(dt * 1e3, 3), "grad_steps_per_s": round(T / dt, 1), "loss": loss}
+    if torch_iters:
+        rate_t, loss_t = torch_reference_rate([cr], [target], T, torch_iters)
+        out.update({"grad_steps_per_s_torch_reference": round(rate_t, 1),
+                    "ratio_vs_torch_reference": round(T / dt / rate_t, 2)})
+    return out
+
+
 def load_traffic(workload_tag, kernel):
     """Measured HBM bytes per launch from committed rocprofv3 PMC summaries, if present."""
     path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload_tag}.json")
@@ -255,7 +287,8 @@ def main():
     ap.add_argument("--radius", type=int, default=26)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="step2048", choices=["step2048", "batched-forward", "batched-grad"],
+    ap.add_argument("--workload", default="step2048",
+                    choices=["step2048", "batched-forward", "batched-grad", "rollout"],
                     help="step2048: the headline 2048^2 C3K9 step; the batched workloads print their own line")
     ap.add_argument("--iters", type=int, default=50, help="batched-grad: Adam iterations")
     ap.add_argument("--plain-iters", type=int, default=0, help="batched-grad: iterations of the plain loop")
@@ -276,6 +309,9 @@ def main():
         print(json.dumps({"workload": "batched-grad", **batched_grad(pkg, iters=args.iters, torch_iters=args.torch_iters,
                                                                       plain_iters=args.plain_iters or None)}),
               flush=True)
+        return
+    if args.workload == "rollout":
+        print(json.dumps({"workload": "rollout", **rollout_grad(pkg, torch_iters=args.torch_iters)}), flush=True)
         return
     if args.workload == "batched-forward":
         print(json.dumps({"workload": "batched-forward", **batched_forward(pkg, steps=args.steps)}), flush=True)
