@@ -151,7 +151,7 @@ k_growth_adj_row(const CreatureDev* __restrict__ descs, const float2* __restrict
   fft_run<L, E, ROWS, false, false, kTwN, 0, true>(v, lds, t, lane, twl);
   __syncthreads();  // orders the red[] writes above before the reduction
   for (int h = NT / 2; h > 0; h >>= 1) {
-    if (threadIdx.x < h)
+    if ((int)threadIdx.x < h)
 #pragma unroll
       for (int j = 0; j < 3; ++j) red[j][threadIdx.x] += red[j][threadIdx.x + h];
     __syncthreads();
@@ -341,7 +341,7 @@ k_mse_grad(const CreatureDev* __restrict__ descs, int sfin, int gout) {
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int h = kLossThreads / 2; h > 0; h >>= 1) {
-    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
     __syncthreads();
   }
   if (threadIdx.x == 0) d.lossp[blockIdx.x] = red[0];

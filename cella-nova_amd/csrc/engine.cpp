@@ -846,7 +846,8 @@ static cn_status ensure_render(cn_engine* e) {
     const cn_status st = dalloc(e, c.allocs, &p, c.plane() * 4);
     if (st != CN_OK) return st;
     c.dev.render = (unsigned int*)p;
-    if (upload_desc(e, (int)i) != CN_OK) return CN_ERR_HIP;
+    const cn_status su = upload_desc(e, (int)i);
+    if (su != CN_OK) return su;
   }
   return CN_OK;
 }

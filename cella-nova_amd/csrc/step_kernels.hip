@@ -401,7 +401,7 @@ k_mass(const CreatureDev* __restrict__ descs, int slot, double2* __restrict__ pa
   sc[threadIdx.x] = comp;
   __syncthreads();
   for (int h = kMassThreads / 2; h > 0; h >>= 1) {
-    if (threadIdx.x < h) {
+    if ((int)threadIdx.x < h) {
       double a = ss[threadIdx.x], ca = sc[threadIdx.x] + sc[threadIdx.x + h];
       neumaier_add(a, ca, ss[threadIdx.x + h]);
       ss[threadIdx.x] = a;
