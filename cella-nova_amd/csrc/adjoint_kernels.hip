@@ -58,11 +58,11 @@ k_diffusion_adj(const CreatureDev* __restrict__ descs, int sin, int gin, int N) 
   const int gx = xl & mask;
   const bool store = l >= 2 && l < 2 + OW && xl < N;
   const size_t plane = (size_t)N * N;
-  const float* A = d.A + c * plane + gx;
-  const float* s = slot_ptr(d, sin) + c * plane + gx;
-  const float* g = d.grad + ((size_t)gin * d.C + c) * plane + gx;
-  float* dsd = d.dsd + c * plane + xl;
-  float* gaff = d.gaff + c * plane + xl;
+  gptr<const float> A = to_global((const float*)d.A) + c * plane + gx;
+  gptr<const float> s = slot_ptr(d, sin) + c * plane + gx;
+  gptr<const float> g = to_global((const float*)d.grad) + ((size_t)gin * d.C + c) * plane + gx;
+  gptr<float> dsd = to_global(d.dsd) + c * plane + xl;
+  gptr<float> gaff = to_global(d.gaff) + c * plane + xl;
   const float temp = d.temp;
 
   // All rows of the tile are loaded up front; step j handles q row yq = y0 - 1 + j and from
@@ -122,8 +122,8 @@ k_growth_adj_row(const CreatureDev* __restrict__ descs, const float2* __restrict
   const int y = blockIdx.x * ROWS + lane;
   const float mu = d.mu[k], is2 = d.inv_s2[k], wk = d.w[k], sg = d.sigma[k];
   const int c = d.dst_of[k];
-  const float2* u = reinterpret_cast<const float2*>(d.u + ((size_t)k * N + y) * N);
-  const float2* ga = reinterpret_cast<const float2*>(d.gaff + ((size_t)c * N + y) * N);
+  gptr<const float2> u = c64((const void*)(d.u + ((size_t)k * N + y) * N));
+  gptr<const float2> ga = c64((const void*)(d.gaff + ((size_t)c * N + y) * N));
   float2 v[E];
   float smu = 0.f, ssg = 0.f, sw = 0.f;
 #pragma unroll
@@ -158,7 +158,7 @@ k_growth_adj_row(const CreatureDev* __restrict__ descs, const float2* __restrict
   }
   if (threadIdx.x < 3) d.partial[((size_t)blockIdx.x * d.K + k) * 3 + threadIdx.x] = red[threadIdx.x][0];
   r2c_finish<N, E, ROWS, true>(v, lds, t, lane, twl + TF::size,
-                               reinterpret_cast<float2*>(d.specg) + ((size_t)k * N + y) * L, true);
+                               c64(d.specg) + ((size_t)k * N + y) * L, true);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -189,7 +189,7 @@ k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) 
   // acc += v * conj(K^_k) at this thread's positions (compact real copy when available)
   auto accumulate = [&](int k, const float2* v, float2* acc) {
     if (d.ksym[k]) {
-      const float* kr = d.kreal + (size_t)k * (HALF + 1) * HALF;
+      gptr<const float> kr = to_global(d.kreal) + (size_t)k * (HALF + 1) * HALF;
 #pragma unroll
       for (int i = 0; i < E; ++i) {
         const int y = out_pos<L, E, false>(t, i);
@@ -197,7 +197,7 @@ k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) 
         acc[i] = make_float2(acc[i].x + v[i].x * r, acc[i].y + v[i].y * r);
       }
     } else {
-      const float2* kh = reinterpret_cast<const float2*>(d.khalf) + (size_t)k * N * HALF;
+      gptr<const float2> kh = c64(d.khalf) + (size_t)k * N * HALF;
 #pragma unroll
       for (int i = 0; i < E; ++i) {
         const int y = out_pos<L, E, false>(t, i);
@@ -211,8 +211,8 @@ k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) 
   for (int kk = kb; kk < ke; ++kk) {
     const int k = d.src_k[kk];
     t = launder(t);
-    const float2* sp = reinterpret_cast<const float2*>(d.specg) + (size_t)k * N * HALF;
-    const float2* kh = reinterpret_cast<const float2*>(d.khalf) + (size_t)k * N * HALF;
+    gptr<const float2> sp = c64((const void*)d.specg) + (size_t)k * N * HALF;
+    gptr<const float2> kh = c64(d.khalf) + (size_t)k * N * HALF;
     float2 v[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) v[i] = sp[(size_t)in_pos<L, E, false>(t, i) * HALF + x];
@@ -227,7 +227,7 @@ k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) 
       }
       __syncthreads();
       if (lane == 0) {
-        const float2* kn = reinterpret_cast<const float2*>(d.knyq) + (size_t)k * N;
+        gptr<const float2> kn = c64(d.knyq) + (size_t)k * N;
 #pragma unroll
         for (int i = 0; i < E; ++i) {
           const int y = out_pos<L, E, false>(t, i);
@@ -248,7 +248,7 @@ k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) 
     }
   }
   fft_run<L, E, TC, true, true, kTwN, 0, true>(acc, lds, t, lane, twl + TF::size);
-  float2* out = reinterpret_cast<float2*>(d.spec) + (size_t)c * N * HALF;
+  gptr<float2> out = c64(d.spec) + (size_t)c * N * HALF;
 #pragma unroll
   for (int i = 0; i < E; ++i) out[(size_t)out_pos<L, E, true>(t, i) * HALF + x] = acc[i];
 }
@@ -274,11 +274,11 @@ k_row_inv_add(const CreatureDev* __restrict__ descs, int gout, const float2* __r
   SplitTw<N>::fill(twl + TI::size, tw, threadIdx.x, blockDim.x);
   __syncthreads();
   float2 v[E];
-  c2r_start<N, E, true>(reinterpret_cast<const float2*>(d.spec) + ((size_t)c * N + y) * L, t, twl + TI::size, v);
+  c2r_start<N, E, true>(c64((const void*)d.spec) + ((size_t)c * N + y) * L, t, twl + TI::size, v);
   fft_run<L, E, ROWS, false, true, kTwN, 0, true>(v, lds, t, lane, twl);
   const size_t row = ((size_t)c * N + y) * N;
-  const float2* dd = reinterpret_cast<const float2*>(d.dsd + row);
-  float2* o = reinterpret_cast<float2*>(d.grad + (size_t)gout * d.C * N * N + row);
+  gptr<const float2> dd = c64((const void*)(d.dsd + row));
+  gptr<float2> o = c64((void*)(d.grad + (size_t)gout * d.C * N * N + row));
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const int n = out_pos<L, E, true>(t, i);
@@ -328,12 +328,13 @@ __global__ void __launch_bounds__(kLossThreads)
 k_mse_grad(const CreatureDev* __restrict__ descs, int sfin, int gout) {
   const CreatureDev& d = descs[blockIdx.z];
   const size_t n = (size_t)d.C * d.N * d.N;
-  const float* x = slot_ptr(d, sfin);
-  float* g = d.grad + (size_t)gout * n;
+  gptr<const float> x = slot_ptr(d, sfin);
+  gptr<const float> tg = to_global((const float*)d.target);
+  gptr<float> g = to_global(d.grad) + (size_t)gout * n;
   const float scale = 2.0f / (float)n;
   float acc = 0.f;
   for (size_t i = (size_t)blockIdx.x * kLossThreads + threadIdx.x; i < n; i += (size_t)kLossBlocks * kLossThreads) {
-    const float r = x[i] - d.target[i];
+    const float r = x[i] - tg[i];
     g[i] = scale * r;
     acc += r * r;
   }

@@ -75,7 +75,7 @@ k_gen_rows(const CreatureDev* __restrict__ desc, const RingParams* __restrict__ 
   SplitTw<N>::fill(twl + TF::size, tw, threadIdx.x, blockDim.x);
   __syncthreads();
   fft_run<L, E, ROWS, false, false, kTwN, 0, true>(v, lds, t, lane, twl);
-  r2c_finish<N, E, ROWS, true>(v, lds, t, lane, twl + TF::size, reinterpret_cast<float2*>(d.uhat) + (size_t)row * L,
+  r2c_finish<N, E, ROWS, true>(v, lds, t, lane, twl + TF::size, c64(d.uhat) + (size_t)row * L,
                                valid);
 }
 
@@ -89,7 +89,7 @@ k_gen_cols(const CreatureDev* __restrict__ desc, const float2* __restrict__ tw) 
   const int lane = threadIdx.x % TC;
   const int t = threadIdx.x / TC;
   const int x = x0 + lane;
-  const float2* src = reinterpret_cast<const float2*>(d.uhat) + (size_t)k * N * HALF;
+  gptr<const float2> src = c64((const void*)d.uhat) + (size_t)k * N * HALF;
   float2 v[E];
 #pragma unroll
   for (int i = 0; i < E; ++i) v[i] = src[(size_t)in_pos<L, E, false>(t, i) * HALF + x];
@@ -100,9 +100,9 @@ k_gen_cols(const CreatureDev* __restrict__ desc, const float2* __restrict__ tw) 
   fft_run<L, E, TC, true, false, kTwN, 0, true>(v, lds, t, lane, twl);
 
   const float sc = 1.0f / ((float)N * (float)N);  // exact: N is a power of two
-  float2* kh = const_cast<float2*>(reinterpret_cast<const float2*>(d.khalf)) + (size_t)k * N * HALF;
-  float2* kn = const_cast<float2*>(reinterpret_cast<const float2*>(d.knyq)) + (size_t)k * N;
-  float* kr = const_cast<float*>(d.kreal) + (size_t)k * (HALF + 1) * HALF;
+  gptr<float2> kh = c64(const_cast<void*>(d.khalf)) + (size_t)k * N * HALF;
+  gptr<float2> kn = c64(const_cast<void*>(d.knyq)) + (size_t)k * N;
+  gptr<float> kr = to_global(const_cast<float*>(d.kreal)) + (size_t)k * (HALF + 1) * HALF;
   if (x0 == 0) {
     // Column 0 holds two real columns packed as (DC column) + i (Nyquist column): split its
     // transform F into A = (F + conj F(-y)) / 2 and B = (F - conj F(-y)) / (2i).
@@ -138,7 +138,7 @@ __global__ void k_gen_seed(const CreatureDev* __restrict__ desc, int slot, uint3
   const CreatureDev& d = *desc;
   const size_t P = (size_t)d.N * d.N;
   const size_t total = (size_t)d.C * P;
-  float* s = slot_ptr(d, slot);
+  gptr<float> s = slot_ptr(d, slot);
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
     const uint32_t c = (uint32_t)(idx / P);
     const uint32_t y = (uint32_t)((idx % P) / d.N), x = (uint32_t)(idx % d.N);

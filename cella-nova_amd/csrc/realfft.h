@@ -31,7 +31,7 @@ struct SplitTw {
 // LT: `tw` is a SplitTw<N> table in LDS, else the global table.
 template <int N, int E, int ROWS, bool LT = false>
 __device__ __forceinline__ void r2c_finish(const float2* v, float2* lds, int t, int lane,
-                                           const float2* __restrict__ tw, float2* out, bool store) {
+                                           const float2* __restrict__ tw, gptr<float2> out, bool store) {
   constexpr int L = N / 2;
   using LM = LdsMap<L, ROWS, false>;
   constexpr bool kWave = wave_rows<L, E>();
@@ -75,7 +75,7 @@ __device__ __forceinline__ void r2c_finish(const float2* v, float2* lds, int t, 
 // N-point real inverse of X.  Split in two so that callers can issue the loads of the next
 // row early (software prefetch).
 template <int N, int E>
-__device__ __forceinline__ void c2r_load(const float2* __restrict__ X, int t, float2* xp, float2* xq) {
+__device__ __forceinline__ void c2r_load(gptr<const float2> X, int t, float2* xp, float2* xq) {
   constexpr int L = N / 2;
 #pragma unroll
   for (int i = 0; i < E; ++i) {
@@ -106,7 +106,7 @@ __device__ __forceinline__ void c2r_form(const float2* xp, const float2* xq, int
 // partner X[L-p] from the other lanes through `lds` (the transform's exchange buffer), ordered
 // at wavefront level.  Halves the row's global loads against c2r_load.
 template <int N, int E>
-__device__ __forceinline__ void c2r_load_own(const float2* __restrict__ X, int t, float2* xp) {
+__device__ __forceinline__ void c2r_load_own(gptr<const float2> X, int t, float2* xp) {
   constexpr int L = N / 2;
 #pragma unroll
   for (int i = 0; i < E; ++i) xp[i] = X[in_pos<L, E, true>(t, i)];
@@ -126,7 +126,7 @@ __device__ __forceinline__ void c2r_partner(const float2* xp, float2* lds, int t
 }
 
 template <int N, int E, bool LT = false>
-__device__ __forceinline__ void c2r_start(const float2* __restrict__ X, int t, const float2* __restrict__ tw,
+__device__ __forceinline__ void c2r_start(gptr<const float2> X, int t, const float2* __restrict__ tw,
                                           float2* v) {
   float2 xp[E], xq[E];
   c2r_load<N, E>(X, t, xp, xq);

@@ -37,7 +37,7 @@ k_row_fwd(const CreatureDev* __restrict__ descs, int sin, const float2* __restri
   const bool valid = row < nrows;
   if (!valid) row = nrows - 1;
 
-  const float2* in = reinterpret_cast<const float2*>(slot_ptr(d, sin) + (size_t)row * N);
+  gptr<const float2> in = (gptr<const float2>)(slot_ptr(d, sin) + (size_t)row * N);
   float2 v[E];
 #pragma unroll
   for (int i = 0; i < E; ++i) v[i] = in[in_pos<L, E, false>(t, i)];
@@ -54,7 +54,7 @@ k_row_fwd(const CreatureDev* __restrict__ descs, int sin, const float2* __restri
 
   // Split the packed transform Z of z[n] = x[2n] + i x[2n+1] into the real row's spectrum.
   r2c_finish<N, E, ROWS, true>(v, lds, t, lane, twl + TF::size,
-                               reinterpret_cast<float2*>(d.spec) + (size_t)row * L, valid);
+                               c64(d.spec) + (size_t)row * L, valid);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -76,7 +76,7 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
   int t = threadIdx.x / TC;
   const int x = x0 + lane;
 
-  const float2* sp = reinterpret_cast<const float2*>(d.spec) + (size_t)c * N * HALF;
+  gptr<const float2> sp = c64((const void*)d.spec) + (size_t)c * N * HALF;
   float2 v[E];
 #pragma unroll
   for (int i = 0; i < E; ++i) v[i] = sp[(size_t)in_pos<L, E, false>(t, i) * HALF + x];
@@ -85,17 +85,17 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
   // flight while kernel k's inverse transform runs (the first ones during the forward one).
   // Kernels with a real spectrum that is even in ky are read from the compact real copy
   // (a quarter of the bytes); the product is the same.
-  const float2* khalf = reinterpret_cast<const float2*>(d.khalf);
+  gptr<const float2> khalf = c64(d.khalf);
   auto load_kernel = [&](int k, float2* out) {
     if (d.ksym[k]) {
-      const float* kr = d.kreal + (size_t)k * (HALF + 1) * HALF;
+      gptr<const float> kr = to_global(d.kreal) + (size_t)k * (HALF + 1) * HALF;
 #pragma unroll
       for (int i = 0; i < E; ++i) {
         const int y = out_pos<L, E, false>(t, i);
         out[i] = make_float2(kr[kreal_index(N, y <= HALF ? y : N - y, x)], 0.0f);
       }
     } else {
-      const float2* kh = khalf + (size_t)k * N * HALF;
+      gptr<const float2> kh = khalf + (size_t)k * N * HALF;
 #pragma unroll
       for (int i = 0; i < E; ++i) out[i] = kh[(size_t)out_pos<L, E, false>(t, i) * HALF + x];
     }
@@ -133,8 +133,8 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
       }
       __syncthreads();
       if (lane == 0) {
-        const float2* kh = khalf + (size_t)k * N * HALF;
-        const float2* kn = reinterpret_cast<const float2*>(d.knyq) + (size_t)k * N;
+        gptr<const float2> kh = khalf + (size_t)k * N * HALF;
+        gptr<const float2> kn = c64(d.knyq) + (size_t)k * N;
         constexpr int G4 = 2;
 #pragma unroll
         for (int i0 = 0; i0 < E; i0 += G4) {
@@ -162,7 +162,7 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
       }
     }
     fft_run<L, E, TC, true, true, kTwN, 0, true>(w, lds, t, lane, twl + TF::size);
-    float2* uo = reinterpret_cast<float2*>(d.uhat) + (size_t)k * N * HALF;
+    gptr<float2> uo = c64(d.uhat) + (size_t)k * N * HALF;
 #pragma unroll
     for (int i = 0; i < E; ++i) uo[(size_t)out_pos<L, E, true>(t, i) * HALF + x] = w[i];
   }
@@ -195,7 +195,7 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
     for (int i = 0; i < E; ++i) acc[i] = make_float2(0.f, 0.f);
     float wsum = 0.f;
     const int kb = d.dst_off[c], ke = d.dst_off[c + 1];
-    const float2* uhat = reinterpret_cast<const float2*>(d.uhat);
+    gptr<const float2> uhat = c64((const void*)d.uhat);
     // Each kernel's row is prefetched one kernel ahead (loads in flight during the transform).
     // A row held by one wavefront loads only its own elements and exchanges the partners.
     // With 16 values per thread the prefetch registers would spill; the row is loaded at the
@@ -236,14 +236,14 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
         const int n = out_pos<L, E, true>(t, i);
         const float u0 = v[i].x, u1 = v[i].y;  // u[2n], u[2n+1]
         if constexpr (WRITE_U) {
-          reinterpret_cast<float2*>(d.u + ((size_t)k * N + y) * N)[n] = v[i];
+          ((gptr<float2>)to_global(d.u + ((size_t)k * N + y) * N))[n] = v[i];
         }
         const float d0 = u0 - mu, d1 = u1 - mu;
         acc[i].x = fmaf(w2, __builtin_amdgcn_exp2f(d0 * (d0 * cexp)), acc[i].x);
         acc[i].y = fmaf(w2, __builtin_amdgcn_exp2f(d1 * (d1 * cexp)), acc[i].y);
       }
     }
-    float2* Ao = reinterpret_cast<float2*>(d.A + ((size_t)c * N + y) * N);
+    gptr<float2> Ao = (gptr<float2>)to_global(d.A + ((size_t)c * N + y) * N);
     const int t = launder(t0);
 #pragma unroll
     for (int i = 0; i < E; ++i) {
@@ -325,12 +325,12 @@ k_diffusion(const CreatureDev* __restrict__ descs, int sin, int sout, int N, int
   const int gx = xl & mask;
   const bool store = l >= 2 && l < 2 + OW && xl < N;
   const size_t plane = (size_t)N * N;
-  const float* A = d.A + c * plane + gx;
-  const float* s = slot_ptr(d, sin) + c * plane + gx;
-  float* o = slot_ptr(d, sout) + c * plane + xl;
+  gptr<const float> A = to_global((const float*)d.A) + c * plane + gx;
+  gptr<const float> s = slot_ptr(d, sin) + c * plane + gx;
+  gptr<float> o = slot_ptr(d, sout) + c * plane + xl;
   // Render epilogue (render.rs:145-153, compute_1024.wgsl:429-443): channel c < 3 writes its
   // byte of the packed 0x00RRGGBB pixel (little endian: R is byte 2, B byte 0).
-  unsigned char* rgb = (render && c < 3 && d.render) ? reinterpret_cast<unsigned char*>(d.render) : nullptr;
+  gptr<unsigned char> rgb = (render && c < 3 && d.render) ? to_global(reinterpret_cast<unsigned char*>(d.render)) : nullptr;
 
   // All rows of the tile are loaded up front (independent loads in flight together), then
   // step j computes q on row y0 - 1 + j; from j = 2 on it also finishes output row y0 + j - 2.
@@ -392,7 +392,7 @@ k_mass(const CreatureDev* __restrict__ descs, int slot, double2* __restrict__ pa
   double2* dst = partial + ((size_t)blockIdx.z * kMaxChannels + c) * nblk + blockIdx.x;
   if (c >= d.C) return;
   const size_t plane = (size_t)d.N * d.N;
-  const float* s = slot_ptr(d, slot) + c * plane;
+  gptr<const float> s = slot_ptr(d, slot) + c * plane;
   double sum = 0.0, comp = 0.0;
   for (size_t i = (size_t)blockIdx.x * kMassThreads + threadIdx.x; i < plane; i += (size_t)nblk * kMassThreads)
     neumaier_add(sum, comp, (double)s[i]);

@@ -102,8 +102,30 @@ inline size_t kreal_index(int N, int ky, int kx) {
 }
 
 #ifdef __HIPCC__
-__device__ __forceinline__ float* slot_ptr(const CreatureDev& d, int s) {
-  return d.state + (size_t)s * d.C * d.N * d.N;
+// Global-memory views of the descriptor's device pointers.  Accesses through a generic pointer
+// loaded from the descriptor compile to FLAT instructions, which count on the LDS/scalar-memory
+// counter as well as on the vector-memory one: a wait for an LDS exchange or an LDS twiddle
+// then also waits for every device-memory load and store in flight (prefetched spectra, the
+// previous result's stores).  Address space 1 makes them global loads and stores, counted on
+// the vector-memory counter only.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using gptr = T __attribute__((address_space(1)))*;
+#else
+template <class T>
+using gptr = T*;  // host pass: kernel bodies are parsed there but never generated
+#endif
+template <class T>
+__device__ __forceinline__ gptr<T> to_global(T* p) {
+  return (gptr<T>)p;
+}
+__device__ __forceinline__ gptr<float> slot_ptr(const CreatureDev& d, int s) {
+  return to_global(d.state + (size_t)s * d.C * d.N * d.N);
+}
+// Complex (float2) views of the spectrum buffers.
+__device__ __forceinline__ gptr<float2> c64(void* p) { return to_global(reinterpret_cast<float2*>(p)); }
+__device__ __forceinline__ gptr<const float2> c64(const void* p) {
+  return to_global(reinterpret_cast<const float2*>(p));
 }
 #endif
 
