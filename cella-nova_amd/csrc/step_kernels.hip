@@ -115,13 +115,23 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
   // Per kernel, lane 0 splits its spectrum F into A = FFT(DC column) and B = FFT(Nyquist
   // column) through the exchange buffer (free between transforms), multiplies each by its own
   // kernel column and re-packs A K_dc + i B K_nyq.
+  //
+  // The first product is formed before the loop: the entry path then has no load of nk in
+  // flight at the loop head, and the wait for a later kernel's prefetch (issued before the
+  // previous kernel's stores) leaves those stores in flight.  Formed only inside the loop, the
+  // head's wait would also cover the entry path, where the prefetch is the newest memory
+  // operation, and would wait for everything.
   const bool edge = (x0 == 0);
+  float2 w[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) w[i] = c_mul(v[i], nk[i]);
   for (int kk = kb; kk < ke; ++kk) {
     const int k = d.src_k[kk];
     t = launder(t);
-    float2 w[E];
+    if (kk > kb) {
 #pragma unroll
-    for (int i = 0; i < E; ++i) w[i] = c_mul(v[i], nk[i]);
+      for (int i = 0; i < E; ++i) w[i] = c_mul(v[i], nk[i]);
+    }
     if (kk + 1 < ke) load_kernel(d.src_k[kk + 1], nk);
     if (edge) {
       // Lane 0 splits the packed DC/Nyquist column; its kernel factors (khalf column 0, knyq)
