@@ -156,7 +156,7 @@ k_growth_adj_row(const CreatureDev* __restrict__ descs, const float2* __restrict
       for (int j = 0; j < 3; ++j) red[j][threadIdx.x] += red[j][threadIdx.x + h];
     __syncthreads();
   }
-  if (threadIdx.x < 3) d.partial[((size_t)blockIdx.x * d.K + k) * 3 + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < 3) to_global(d.partial)[((size_t)blockIdx.x * d.K + k) * 3 + threadIdx.x] = red[threadIdx.x][0];
   r2c_finish<N, E, ROWS, true>(v, lds, t, lane, twl + TF::size,
                                c64(d.specg) + ((size_t)k * N + y) * L, true);
 }
@@ -345,7 +345,7 @@ k_mse_grad(const CreatureDev* __restrict__ descs, int sfin, int gout) {
     if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
     __syncthreads();
   }
-  if (threadIdx.x == 0) d.lossp[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) to_global(d.lossp)[blockIdx.x] = red[0];
 }
 
 // ---------------------------------------------------------------------------------------
