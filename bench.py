@@ -300,8 +300,8 @@ def main():
                     help="also run a device-to-device copy of this many MiB (PMC calibration only)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this engine runs on one GPU only; multi-GPU runs are not part of this build "
-              "(creature batches shard without communication, see DESIGN.md)", file=sys.stderr)
+        print("bench.py: this benchmark runs on one GPU only (--gpus 1); creature batches shard across "
+              "GPUs without communication, one process and one CreatureBatch per GPU", file=sys.stderr)
         sys.exit(2)
 
     pkg = ge.load_package()
