@@ -39,18 +39,39 @@ def algorithmic_bytes_per_cell(C, K, compact=0):
             "row_inv_growth": 4 * K + 4 * C, "diffusion": 12 * C}
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def cpu_baseline(sample_steps):
-    """ml-rs-example restatement (oracle/cpu_ref.cpp, f32, one thread) on 1024^2 C3K9 R=13."""
+    """ml-rs-example restatement (oracle/cpu_ref.cpp, f32, one thread) on 1024^2 C3K9 R=13,
+    pinned to one core of the host (the reference's CPU example is single-threaded)."""
     orc = ge.load_oracle()
     pkg = ge.load_package()
     cr = pkg.baseline_creature(1024, num_channels=3, radius=13, seed=0)
-    t0 = time.perf_counter()
-    orc.run_f32(cr["state"], cr["kernels_fft"], cr["src"], cr["dst"], cr["mu"], cr["sigma"], cr["weights"],
-                steps=sample_steps)
-    dt = time.perf_counter() - t0
+    allowed = os.sched_getaffinity(0) if hasattr(os, "sched_getaffinity") else None
+    core = min(allowed) if allowed else None
+    if core is not None:
+        os.sched_setaffinity(0, {core})
+    try:
+        t0 = time.perf_counter()
+        orc.run_f32(cr["state"], cr["kernels_fft"], cr["src"], cr["dst"], cr["mu"], cr["sigma"], cr["weights"],
+                    steps=sample_steps)
+        dt = time.perf_counter() - t0
+    finally:
+        if core is not None:
+            os.sched_setaffinity(0, allowed)
+    pin = f"pinned to core {core} of {len(allowed)}" if core is not None else "unpinned"
     return {"value": 1024 * 1024 * sample_steps / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
             "sample": f"{sample_steps} steps of 1024x1024 C3K9 R=13 (ml-rs-example restatement, f32, "
-                      f"1 thread, {dt:.1f} s)"}
+                      f"1 thread {pin}, {dt:.1f} s; host {_cpu_model()}, nproc {os.cpu_count()})"}
 
 
 def _adam(p, g, m, v, t, lr=5e-3, b1=0.9, b2=0.999, eps=1e-8):
