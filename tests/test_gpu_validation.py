@@ -20,8 +20,8 @@ DEV = "cuda:0"
 TOL = 1e-5
 
 
-def _per_step_errors(cn, orc, n, R, steps):
-    cr = cn.baseline_creature(n, radius=R, seed=0)
+def _per_step_errors(cn, orc, n, R, steps, seed=0):
+    cr = cn.baseline_creature(n, radius=R, seed=seed)
     eng = cn.make_engine(cr)
     kf, src, dst, mu, sg, w = torch_ref.creature_tensors(cr, DEV)
     free = torch.tensor(cr["state"], dtype=torch.float64, device=DEV)
@@ -41,9 +41,10 @@ def _per_step_errors(cn, orc, n, R, steps):
     return np.array(errs), free_err, drift
 
 
-def test_config1_1024_c3k9_100_steps(cn, orc):
-    errs, free_err, drift = _per_step_errors(cn, orc, 1024, 13, 100)
-    print(f"1024^2 C3K9 R=13, 100 steps: per-step rel L-inf vs FP64 max {errs.max():.2e} mean {errs.mean():.2e}; "
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_config1_1024_c3k9_100_steps(cn, orc, seed):
+    errs, free_err, drift = _per_step_errors(cn, orc, 1024, 13, 100, seed)
+    print(f"1024^2 C3K9 R=13 seed {seed}, 100 steps: per-step rel L-inf vs FP64 max {errs.max():.2e} mean {errs.mean():.2e}; "
           f"free-running after 100 steps {free_err:.2e}; FP64 mass drift {drift:.2e}")
     assert errs.max() <= TOL
     assert drift <= 1e-6
