@@ -102,9 +102,10 @@ __device__ __forceinline__ void c2r_form(const float2* xp, const float2* xq, int
   }
 }
 
-// Single-wavefront rows (wave_rows<N/2, E>()): each lane loads only its own X[p] and takes the
-// partner X[L-p] from the other lanes through `lds` (the transform's exchange buffer), ordered
-// at wavefront level.  Halves the row's global loads against c2r_load.
+// Each lane loads only its own X[p] and takes the partner X[L-p] from the other lanes through
+// `lds` (the transform's exchange buffer), ordered at wavefront level for single-wavefront rows
+// (wave_rows<N/2, E>()) and by block barriers otherwise.  Halves the row's global loads (and
+// the registers of a prefetched row) against c2r_load.
 template <int N, int E>
 __device__ __forceinline__ void c2r_load_own(gptr<const float2> X, int t, float2* xp) {
   constexpr int L = N / 2;
@@ -115,12 +116,12 @@ __device__ __forceinline__ void c2r_load_own(gptr<const float2> X, int t, float2
 template <int N, int E, int ROWS>
 __device__ __forceinline__ void c2r_partner(const float2* xp, float2* lds, int t, int lane, float2* xq) {
   constexpr int L = N / 2;
-  static_assert(wave_rows<L, E>(), "partner exchange through LDS needs one wavefront per row");
+  constexpr bool kWave = wave_rows<L, E>();
   using LM = LdsMap<L, ROWS, false>;
-  xsync<true>();
+  xsync<kWave>();
 #pragma unroll
   for (int i = 0; i < E; ++i) lds[LM::at(in_pos<L, E, true>(t, i), lane)] = xp[i];
-  xsync<true>();
+  xsync<kWave>();
 #pragma unroll
   for (int i = 0; i < E; ++i) xq[i] = lds[LM::at((L - in_pos<L, E, true>(t, i)) & (L - 1), lane)];
 }

@@ -207,18 +207,13 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
     const int kb = d.dst_off[c], ke = d.dst_off[c + 1];
     gptr<const float2> uhat = c64((const void*)d.uhat);
     // Each kernel's row is prefetched one kernel ahead (loads in flight during the transform).
-    // A row held by one wavefront loads only its own elements and exchanges the partners.
-    // With 16 values per thread the prefetch registers would spill; the row is loaded at the
-    // start of its iteration instead (a one-wavefront row has no block barrier to wait on).
+    // Every lane loads only its own elements and takes the partners through LDS.  With 16
+    // values per thread the prefetch registers would spill; the row is loaded at the start of
+    // its iteration instead (a one-wavefront row has no block barrier to wait on).
     constexpr bool kWave = wave_rows<L, E>();
     constexpr bool kPrefetch = !(kWave && E >= 16);
     float2 xp[E], xq[E];
-    if (kPrefetch && kb < ke) {
-      if constexpr (kWave)
-        c2r_load_own<N, E>(uhat + ((size_t)d.dst_k[kb] * N + y) * L, t0, xp);
-      else
-        c2r_load<N, E>(uhat + ((size_t)d.dst_k[kb] * N + y) * L, t0, xp, xq);
-    }
+    if (kPrefetch && kb < ke) c2r_load_own<N, E>(uhat + ((size_t)d.dst_k[kb] * N + y) * L, t0, xp);
     for (int kk = kb; kk < ke; ++kk) {
       const int k = d.dst_k[kk];
       const int t = launder(t0);
@@ -232,14 +227,9 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
       wsum += wk;
       float2 v[E];
       if constexpr (!kPrefetch) c2r_load_own<N, E>(uhat + ((size_t)k * N + y) * L, t, xp);
-      if constexpr (kWave) c2r_partner<N, E, ROWS>(xp, lds, t, lane, xq);
+      c2r_partner<N, E, ROWS>(xp, lds, t, lane, xq);
       c2r_form<N, E, true>(xp, xq, t, twl + TI::size, v);
-      if (kPrefetch && kk + 1 < ke) {
-        if constexpr (kWave)
-          c2r_load_own<N, E>(uhat + ((size_t)d.dst_k[kk + 1] * N + y) * L, t, xp);
-        else
-          c2r_load<N, E>(uhat + ((size_t)d.dst_k[kk + 1] * N + y) * L, t, xp, xq);
-      }
+      if (kPrefetch && kk + 1 < ke) c2r_load_own<N, E>(uhat + ((size_t)d.dst_k[kk + 1] * N + y) * L, t, xp);
       fft_run<L, E, ROWS, false, true, kTwN, 0, true>(v, lds, t, lane, twl);
 #pragma unroll
       for (int i = 0; i < E; ++i) {
