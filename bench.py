@@ -306,7 +306,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--radius", type=int, default=26)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="step2048",
                     choices=["step2048", "batched-forward", "batched-grad", "rollout"],
