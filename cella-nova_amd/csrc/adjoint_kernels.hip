@@ -287,26 +287,41 @@ k_row_inv_add(const CreatureDev* __restrict__ descs, int gout, const float2* __r
 }
 
 // ---------------------------------------------------------------------------------------
-// Parameter gradients: sum the per-block partials (fixed order) and add them to the
-// accumulated gradients with Kahan compensation.  gparam = [mu | sigma | w] then the
-// compensation terms.
+// Parameter gradients: sum the per-block partials and add them to the accumulated gradients
+// with Kahan compensation.  One wavefront per (kernel, parameter) pair: lane l sums partials
+// l, l + 64, ... (Kahan), then the lanes combine in a fixed xor butterfly with TwoSum, so the
+// result is the same on every run.  gparam = [mu | sigma | w] then the compensation terms.
 // ---------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
+constexpr int kReduceWaves = 4;
+
+__global__ void __launch_bounds__(64 * kReduceWaves)
 k_param_reduce(const CreatureDev* __restrict__ descs, int nblk) {
   const CreatureDev& d = descs[blockIdx.z];
   const int K = d.K;
-  for (int idx = threadIdx.x; idx < 3 * K; idx += blockDim.x) {
-    const int k = idx / 3, j = idx % 3;
-    float s = 0.f, comp = 0.f;
-    for (int b = 0; b < nblk; ++b) {
-      const float yv = d.partial[((size_t)b * K + k) * 3 + j] - comp;
-      const float tt = s + yv;
-      comp = (tt - s) - yv;
-      s = tt;
-    }
-    float* acc = d.gparam + j * K + k;
-    float* cmp = d.gparam + 3 * K + j * K + k;
-    const float yv = s - *cmp;
+  const int idx = (int)blockIdx.x * kReduceWaves + (int)(threadIdx.x >> 6);
+  if (idx >= 3 * K) return;  // a whole wavefront; the kernel has no barriers
+  const int lane = threadIdx.x & 63;
+  const int k = idx / 3, j = idx % 3;
+  gptr<const float> part = to_global((const float*)d.partial);
+  float s = 0.f, comp = 0.f;  // running sum and Kahan compensation (true sum = s - comp)
+  for (int b = lane; b < nblk; b += 64) {
+    const float yv = part[((size_t)b * K + k) * 3 + j] - comp;
+    const float tt = s + yv;
+    comp = (tt - s) - yv;
+    s = tt;
+  }
+  for (int m = 32; m >= 1; m >>= 1) {
+    const float s2 = __shfl_xor(s, m, 64), c2 = __shfl_xor(comp, m, 64);
+    const float t = s + s2;  // TwoSum: s + s2 = t + e exactly
+    const float bv = t - s;
+    const float e = (s - (t - bv)) + (s2 - bv);
+    comp = (comp + c2) - e;
+    s = t;
+  }
+  if (lane == 0) {
+    gptr<float> acc = to_global(d.gparam) + j * K + k;
+    gptr<float> cmp = to_global(d.gparam) + 3 * K + j * K + k;
+    const float yv = (s - comp) - *cmp;
     const float tt = *acc + yv;
     *cmp = (tt - *acc) - yv;
     *acc = tt;
@@ -379,7 +394,8 @@ static hipError_t launch_adj_rows(const StepLaunch& sl, int gout, hipStream_t st
     k_growth_adj_row<N, ER><<<dim3(CF::BLOCKS, sl.max_k, sl.count), CF::THREADS, 0, st>>>(sl.descs, sl.tw);
   } else {
     k_row_inv_add<N, ER><<<dim3(CF::BLOCKS, sl.max_c, sl.count), CF::THREADS, 0, st>>>(sl.descs, gout, sl.tw);
-    k_param_reduce<<<dim3(1, 1, sl.count), 256, 0, st>>>(sl.descs, CF::BLOCKS);
+    k_param_reduce<<<dim3((3 * sl.max_k + kReduceWaves - 1) / kReduceWaves, 1, sl.count), 64 * kReduceWaves, 0, st>>>(
+        sl.descs, CF::BLOCKS);
   }
   return hipSuccess;
 }
