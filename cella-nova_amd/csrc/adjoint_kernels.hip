@@ -40,7 +40,7 @@ __device__ __forceinline__ float hsum3f(float x) {
 // shifts and vertical ones in rolling registers while walking down the rows.
 // ---------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(kAdjThreads)
-k_diffusion_adj(const CreatureDev* __restrict__ descs, int sin, int gin, int N) {
+k_diffusion_adj(const CreatureDev* __restrict__ descs, int sin, int gin, int N, int islot) {
   constexpr int OW = kAdjTW, TH = kAdjTH;
   const CreatureDev& d = descs[blockIdx.z];
   if (d.N != N) return;
@@ -58,7 +58,7 @@ k_diffusion_adj(const CreatureDev* __restrict__ descs, int sin, int gin, int N) 
   const int gx = xl & mask;
   const bool store = l >= 2 && l < 2 + OW && xl < N;
   const size_t plane = (size_t)N * N;
-  gptr<const float> A = to_global((const float*)d.A) + c * plane + gx;
+  gptr<const float> A = to_global((const float*)d.A) + ((size_t)islot * d.C + c) * plane + gx;
   gptr<const float> s = slot_ptr(d, sin) + c * plane + gx;
   gptr<const float> g = to_global((const float*)d.grad) + ((size_t)gin * d.C + c) * plane + gx;
   gptr<float> dsd = to_global(d.dsd) + c * plane + xl;
@@ -111,7 +111,7 @@ k_diffusion_adj(const CreatureDev* __restrict__ descs, int sin, int gin, int N) 
 // ---------------------------------------------------------------------------------------
 template <int N, int ER>
 __global__ void __launch_bounds__((RowCfg<N, ER>::THREADS))
-k_growth_adj_row(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) {
+k_growth_adj_row(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw, int islot) {
   using CF = RowCfg<N, ER>;
   constexpr int L = CF::L, E = CF::E, T = CF::T, ROWS = CF::ROWS, NT = CF::THREADS;
   const CreatureDev& d = descs[blockIdx.z];
@@ -122,7 +122,7 @@ k_growth_adj_row(const CreatureDev* __restrict__ descs, const float2* __restrict
   const int y = blockIdx.x * ROWS + lane;
   const float mu = d.mu[k], is2 = d.inv_s2[k], wk = d.w[k], sg = d.sigma[k];
   const int c = d.dst_of[k];
-  gptr<const float2> u = c64((const void*)(d.u + ((size_t)k * N + y) * N));
+  gptr<const float2> u = c64((const void*)(d.u + (((size_t)islot * d.K + k) * N + y) * N));
   gptr<const float2> ga = c64((const void*)(d.gaff + ((size_t)c * N + y) * N));
   float2 v[E];
   float smu = 0.f, ssg = 0.f, sw = 0.f;
@@ -391,7 +391,7 @@ template <int N, int ER>
 static hipError_t launch_adj_rows(const StepLaunch& sl, int gout, hipStream_t st, bool growth) {
   using CF = RowCfg<N, ER>;
   if (growth) {
-    k_growth_adj_row<N, ER><<<dim3(CF::BLOCKS, sl.max_k, sl.count), CF::THREADS, 0, st>>>(sl.descs, sl.tw);
+    k_growth_adj_row<N, ER><<<dim3(CF::BLOCKS, sl.max_k, sl.count), CF::THREADS, 0, st>>>(sl.descs, sl.tw, sl.islot);
   } else {
     k_row_inv_add<N, ER><<<dim3(CF::BLOCKS, sl.max_c, sl.count), CF::THREADS, 0, st>>>(sl.descs, gout, sl.tw);
     k_param_reduce<<<dim3((3 * sl.max_k + kReduceWaves - 1) / kReduceWaves, 1, sl.count), 64 * kReduceWaves, 0, st>>>(
@@ -400,20 +400,23 @@ static hipError_t launch_adj_rows(const StepLaunch& sl, int gout, hipStream_t st
   return hipSuccess;
 }
 
-// One adjoint step: recompute the forward intermediates of state slot sl.sin (u, A), then
-// grad slot gin -> gout and parameter-gradient accumulation.
+// One adjoint step: (sl.adj_recompute) recompute the forward intermediates of state slot sl.sin
+// (u, A) into intermediate slot sl.islot, then grad slot gin -> gout from those intermediates and
+// parameter-gradient accumulation.
 template <int N>
 static hipError_t launch_adjoint_n(const StepLaunch& sl, int gin, int gout, hipStream_t st) {
-  StepLaunch fw = sl;
-  fw.write_u = true;
-  fw.skip_diffusion = true;
-  fw.ev[0] = fw.ev[1] = fw.ev[2] = fw.ev[3] = fw.ev[4] = nullptr;
-  hipError_t e = launch_step(N, fw, st);
-  if (e != hipSuccess) return e;
+  hipError_t e;
+  if (sl.adj_recompute) {
+    StepLaunch fw = sl;
+    fw.write_u = true;
+    fw.skip_diffusion = true;
+    fw.ev[0] = fw.ev[1] = fw.ev[2] = fw.ev[3] = fw.ev[4] = nullptr;
+    if ((e = launch_step(N, fw, st)) != hipSuccess) return e;
+  }
   const int tiles_x = (N + kAdjTW - 1) / kAdjTW, tiles_y = (N + kAdjTH - 1) / kAdjTH;
   const int waves = kAdjThreads / 64;
   k_diffusion_adj<<<dim3((tiles_x * tiles_y + waves - 1) / waves, sl.max_c, sl.count), kAdjThreads, 0, st>>>(
-      sl.descs, sl.sin, gin, N);
+      sl.descs, sl.sin, gin, N, sl.islot);
   auto rows = [&](bool growth) {
     const int re = growth ? (sl.fwd_e ? sl.fwd_e : DefaultGeom<N>::FWD_E) : (sl.inv_e ? sl.inv_e : DefaultGeom<N>::INV_E);
     switch (re) {

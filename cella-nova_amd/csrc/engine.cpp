@@ -196,6 +196,7 @@ struct Creature {
   int cls = 0, idx = 0;    // size class and position in its descriptor array
   std::vector<void*> allocs;
   bool adj = false;        // adjoint buffers allocated
+  int islots = 1;          // intermediate slots of A (and of u once allocated)
   bool has_target = false;
   size_t plane() const { return (size_t)N * N; }
   size_t slot_elems() const { return (size_t)C * N * N; }
@@ -313,9 +314,28 @@ hipError_t launch_all_steps(cn_engine* e, int sin, int sout, bool render = false
   return hipSuccess;
 }
 
-hipError_t launch_all_adjoint(cn_engine* e, int sin, int gin, int gout) {
+// A step of every class that keeps its intermediates (u with write_u, A) in slot `islot`;
+// skip_diffusion stops before the new state.
+hipError_t launch_all_keep(cn_engine* e, int sin, int sout, bool skip_diffusion, int islot) {
   for (const SizeClass& k : e->cls) {
-    const hipError_t err = launch_adjoint(k.N, class_launch(e, k, sin, sin), gin, gout, e->stream);
+    StepLaunch sl = class_launch(e, k, sin, sout);
+    sl.write_u = true;
+    sl.skip_diffusion = skip_diffusion;
+    sl.islot = islot;
+    const hipError_t err = launch_step(k.N, sl, e->stream);
+    if (err != hipSuccess) return err;
+  }
+  return hipSuccess;
+}
+
+// Adjoint step of state slot `sin` from the intermediates in slot `islot` (recomputed first
+// when `recompute`).
+hipError_t launch_all_adjoint(cn_engine* e, int sin, int gin, int gout, int islot, bool recompute) {
+  for (const SizeClass& k : e->cls) {
+    StepLaunch sl = class_launch(e, k, sin, sin);
+    sl.islot = islot;
+    sl.adj_recompute = recompute;
+    const hipError_t err = launch_adjoint(k.N, sl, gin, gout, e->stream);
     if (err != hipSuccess) return err;
   }
   return hipSuccess;
@@ -421,6 +441,40 @@ cn_status ensure_adjoint(cn_engine* e) {
   return CN_OK;
 }
 
+// Grows every creature's intermediate arrays (A: C planes, u: K planes per slot) to `want`
+// slots; their contents are recomputed before every use, so nothing is copied.
+cn_status ensure_islots(cn_engine* e, int want) {
+  bool sync = false;
+  for (size_t i = 0; i < e->cr.size(); ++i) {
+    Creature& c = e->cr[i];
+    if (c.islots >= want && c.dev.u) continue;
+    if (!sync) {
+      CN_HIP(hipStreamSynchronize(e->stream));
+      sync = true;
+    }
+    const int n = std::max(want, c.islots);
+    for (float** buf : {&c.dev.A, &c.dev.u}) {
+      const size_t planes = (buf == &c.dev.A) ? c.C : c.K;
+      void* p = nullptr;
+      const cn_status st = dalloc(e, c.allocs, &p, (size_t)n * planes * c.plane() * 4);
+      if (st != CN_OK) return st;
+      if (*buf) {
+        auto it = std::find(c.allocs.begin(), c.allocs.end(), (void*)*buf);
+        if (it != c.allocs.end()) {
+          CN_HIP(hipFree(*it));
+          e->bytes -= (size_t)c.islots * planes * c.plane() * 4;
+          c.allocs.erase(it);
+        }
+      }
+      *buf = (float*)p;
+    }
+    c.islots = n;
+    const cn_status s2 = upload_desc(e, (int)i);
+    if (s2 != CN_OK) return s2;
+  }
+  return CN_OK;
+}
+
 // Slot plan of a rollout of T steps with a checkpoint every `ck` steps, starting from slot
 // `start` (0 or 1, left unmodified):
 //   ck[0] = start; ck[j] (1 <= j < nck) = slot 2 + j - 1; forward scratch X0, X1;
@@ -463,10 +517,17 @@ hipError_t record_backward(cn_engine* e, const SlotPlan& p, uint32_t T, uint32_t
   for (int j = p.nck - 1; j >= 0; --j) {
     const uint32_t t0 = (uint32_t)j * ck, t1 = std::min(T, t0 + ck);
     auto state_slot = [&](uint32_t t) { return t == t0 ? p.ckslot(j, start) : p.segslot((int)(t - t0)); };
-    for (uint32_t t = t0 + 1; t < t1; ++t)
-      if ((err = launch_all_steps(e, state_slot(t - 1), state_slot(t))) != hipSuccess) return err;
+    // Recompute the segment from its checkpoint, keeping each step's intermediates (u, A) in
+    // intermediate slot t - t0; the last step stops before its diffusion (its output is the
+    // next checkpoint or the final state, both stored).
+    for (uint32_t t = t0; t < t1; ++t) {
+      const bool last = t + 1 == t1;
+      if ((err = launch_all_keep(e, state_slot(t), last ? state_slot(t) : state_slot(t + 1), last, (int)(t - t0))) !=
+          hipSuccess)
+        return err;
+    }
     for (uint32_t t = t1; t-- > t0;) {
-      if ((err = launch_all_adjoint(e, state_slot(t), g, g ^ 1)) != hipSuccess) return err;
+      if ((err = launch_all_adjoint(e, state_slot(t), g, g ^ 1, (int)(t - t0), false)) != hipSuccess) return err;
       g ^= 1;
     }
   }
@@ -481,6 +542,7 @@ cn_status prepare_rollout(cn_engine* e, uint32_t T, uint32_t ck) {
   if ((st = ensure_adjoint(e)) != CN_OK) return st;
   const SlotPlan p = plan_slots(T, ck);
   if ((st = ensure_slots(e, p.total)) != CN_OK) return st;
+  if ((st = ensure_islots(e, (int)ck)) != CN_OK) return st;
   if (e->ro.T == T && e->ro.ckpt == ck && e->ro.graph && e->ro.fwd) return CN_OK;
   if (e->ro.graph) (void)hipGraphExecDestroy(e->ro.graph);
   if (e->ro.fwd) (void)hipGraphExecDestroy(e->ro.fwd);

@@ -16,6 +16,10 @@ struct StepLaunch {
   bool write_u = false;                // also write the real convolution results to desc.u
   bool skip_diffusion = false;         // stop after the affinity (conv readback)
   bool render = false;                 // diffusion also packs channels 0..2 into desc.render
+  // Slot of the step intermediates written (inverse-row pass) and read (diffusion, adjoint):
+  // the affinity at desc.A + islot*C*N*N and the convolution results at desc.u + islot*K*N*N.
+  int islot = 0;
+  bool adj_recompute = true;           // adjoint: recompute u and A of sin into islot first
   // Launch geometry (0: built-in default): values per thread of the forward (R2C) and
   // inverse (C2R) row FFTs and of the column FFTs, column tile width, diffusion tile height.
   int fwd_e = 0, inv_e = 0, col_e = 0, col_tc = 0, diff_th = 0;

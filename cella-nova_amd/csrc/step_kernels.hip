@@ -183,7 +183,7 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
 // ---------------------------------------------------------------------------------------
 template <int N, int ER, bool WRITE_U>
 __global__ void __launch_bounds__((RowCfg<N, ER>::THREADS)) __attribute__((amdgpu_waves_per_eu(4)))
-k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) {
+k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw, int islot) {
   using CF = RowCfg<N, ER>;
   constexpr int L = CF::L, E = CF::E, T = CF::T, ROWS = CF::ROWS;
   const CreatureDev& d = descs[blockIdx.z];
@@ -236,14 +236,14 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
         const int n = out_pos<L, E, true>(t, i);
         const float u0 = v[i].x, u1 = v[i].y;  // u[2n], u[2n+1]
         if constexpr (WRITE_U) {
-          ((gptr<float2>)to_global(d.u + ((size_t)k * N + y) * N))[n] = v[i];
+          ((gptr<float2>)to_global(d.u + (((size_t)islot * d.K + k) * N + y) * N))[n] = v[i];
         }
         const float d0 = u0 - mu, d1 = u1 - mu;
         acc[i].x = fmaf(w2, __builtin_amdgcn_exp2f(d0 * (d0 * cexp)), acc[i].x);
         acc[i].y = fmaf(w2, __builtin_amdgcn_exp2f(d1 * (d1 * cexp)), acc[i].y);
       }
     }
-    gptr<float2> Ao = (gptr<float2>)to_global(d.A + ((size_t)c * N + y) * N);
+    gptr<float2> Ao = (gptr<float2>)to_global(d.A + (((size_t)islot * d.C + c) * N + y) * N);
     const int t = launder(t0);
 #pragma unroll
     for (int i = 0; i < E; ++i) {
@@ -307,7 +307,7 @@ __device__ __forceinline__ double hsum3(float x) {
 
 template <int TH>
 __global__ void __launch_bounds__(kDiffThreads)
-k_diffusion(const CreatureDev* __restrict__ descs, int sin, int sout, int N, int render) {
+k_diffusion(const CreatureDev* __restrict__ descs, int sin, int sout, int N, int render, int islot) {
   constexpr int OW = kDiffTW;
   const CreatureDev& d = descs[blockIdx.z];
   if (d.N != N) return;
@@ -325,7 +325,7 @@ k_diffusion(const CreatureDev* __restrict__ descs, int sin, int sout, int N, int
   const int gx = xl & mask;
   const bool store = l >= 2 && l < 2 + OW && xl < N;
   const size_t plane = (size_t)N * N;
-  gptr<const float> A = to_global((const float*)d.A) + c * plane + gx;
+  gptr<const float> A = to_global((const float*)d.A) + ((size_t)islot * d.C + c) * plane + gx;
   gptr<const float> s = slot_ptr(d, sin) + c * plane + gx;
   gptr<float> o = slot_ptr(d, sout) + c * plane + xl;
   // Render epilogue (render.rs:145-153, compute_1024.wgsl:429-443): channel c < 3 writes its
@@ -448,9 +448,9 @@ static hipError_t launch_rows(const StepLaunch& sl, hipStream_t st, bool fwd) {
   } else {
     const dim3 g_inv(CF::BLOCKS, sl.max_c, sl.count);
     if (sl.write_u)
-      k_row_inv_growth<N, ER, true><<<g_inv, CF::THREADS, 0, st>>>(sl.descs, sl.tw);
+      k_row_inv_growth<N, ER, true><<<g_inv, CF::THREADS, 0, st>>>(sl.descs, sl.tw, sl.islot);
     else
-      k_row_inv_growth<N, ER, false><<<g_inv, CF::THREADS, 0, st>>>(sl.descs, sl.tw);
+      k_row_inv_growth<N, ER, false><<<g_inv, CF::THREADS, 0, st>>>(sl.descs, sl.tw, sl.islot);
   }
   return hipSuccess;
 }
@@ -487,9 +487,9 @@ static hipError_t launch_step_n(const StepLaunch& sl, hipStream_t st) {
     const int waves = kDiffThreads / 64;
     const dim3 g_dif((tiles_x * tiles_y + waves - 1) / waves, sl.max_c, sl.count);
     switch (th) {
-      case 8: k_diffusion<8><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N, sl.render); break;
-      case 16: k_diffusion<16><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N, sl.render); break;
-      case 32: k_diffusion<32><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N, sl.render); break;
+      case 8: k_diffusion<8><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N, sl.render, sl.islot); break;
+      case 16: k_diffusion<16><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N, sl.render, sl.islot); break;
+      case 32: k_diffusion<32><<<g_dif, kDiffThreads, 0, st>>>(sl.descs, sl.sin, sl.sout, N, sl.render, sl.islot); break;
       default: return hipErrorInvalidValue;
     }
   }

@@ -14,8 +14,9 @@
 //                              ksym[k] = 1
 //   uhat    [K][N][N/2]  c64   row spectra of every convolution result (after the inverse
 //                              column pass); slot kx = 0 packs (DC, Nyquist) as (re, im)
-//   A       [C][N][N]    f32   affinity exp(temp * sum_k w_k G_k) per destination channel
-//   u       [K][N][N]    f32   convolution results (readback and adjoint only)
+//   A       [islots][C][N][N] f32 affinity exp(temp * sum_k w_k G_k) per destination channel;
+//                              slot 0 for stepping, one slot per step of a rollout segment
+//   u       [islots][K][N][N] f32 convolution results (readback and adjoint only)
 //   render  [N][N]       u32   packed 0x00RRGGBB frame of channels 0..2 (allocated on the first
 //                              iterate_and_render; the diffusion pass writes bytes 2, 1, 0)
 // Adjoint buffers (allocated on first rollout):
