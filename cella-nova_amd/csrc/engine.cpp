@@ -494,22 +494,29 @@ SlotPlan plan_slots(uint32_t T, uint32_t ck) {
   return p;
 }
 
+// The steps of the last segment keep their states (segment slots) and intermediates (slot
+// t - tl), so a backward recorded right after it can skip that segment's recompute.
 hipError_t record_forward(cn_engine* e, const SlotPlan& p, uint32_t T, uint32_t ck, int start) {
+  const uint32_t tl = (uint32_t)(p.nck - 1) * ck;  // first step of the last segment
   int in = start;
   for (uint32_t t = 0; t < T; ++t) {
     int out;
     if (t + 1 == T) out = p.fin;
     else if ((t + 1) % ck == 0) out = p.ckslot((int)((t + 1) / ck), start);
+    else if (t + 1 > tl) out = p.segslot((int)(t + 1 - tl));
     else out = p.x0 + (int)(t % 2);
-    const hipError_t err = launch_all_steps(e, in, out);
+    const hipError_t err =
+        t >= tl ? launch_all_keep(e, in, out, false, (int)(t - tl)) : launch_all_steps(e, in, out);
     if (err != hipSuccess) return err;
     in = out;
   }
   return hipSuccess;
 }
 
-// Backward from gradient slot 0 (dL/dfinal); returns the slot holding dL/dstate0.
-hipError_t record_backward(cn_engine* e, const SlotPlan& p, uint32_t T, uint32_t ck, int start, int* gslot) {
+// Backward from gradient slot 0 (dL/dfinal); returns the slot holding dL/dstate0.  last_kept:
+// the last segment's states and intermediates are still those record_forward left.
+hipError_t record_backward(cn_engine* e, const SlotPlan& p, uint32_t T, uint32_t ck, int start, bool last_kept,
+                           int* gslot) {
   hipError_t err;
   for (const SizeClass& k : e->cls)
     if ((err = launch_zero_gparam(k.d_descs, (int)k.members.size(), e->stream)) != hipSuccess) return err;
@@ -520,7 +527,7 @@ hipError_t record_backward(cn_engine* e, const SlotPlan& p, uint32_t T, uint32_t
     // Recompute the segment from its checkpoint, keeping each step's intermediates (u, A) in
     // intermediate slot t - t0; the last step stops before its diffusion (its output is the
     // next checkpoint or the final state, both stored).
-    for (uint32_t t = t0; t < t1; ++t) {
+    for (uint32_t t = t0; t < t1 && !(last_kept && j == p.nck - 1); ++t) {
       const bool last = t + 1 == t1;
       if ((err = launch_all_keep(e, state_slot(t), last ? state_slot(t) : state_slot(t + 1), last, (int)(t - t0))) !=
           hipSuccess)
@@ -555,7 +562,7 @@ cn_status prepare_rollout(cn_engine* e, uint32_t T, uint32_t ck) {
     if (err != hipSuccess) return err;
     for (const SizeClass& k : e->cls)
       if ((err = launch_mse_grad(k.d_descs, (int)k.members.size(), p.fin, 0, e->stream)) != hipSuccess) return err;
-    return record_backward(e, p, T, ck, 0, &gslot);
+    return record_backward(e, p, T, ck, 0, true, &gslot);
   });
   if (st != CN_OK) return st;
   e->ro.T = T;
@@ -1205,7 +1212,8 @@ cn_status cn_rollout_backward(cn_engine* e, const float* const* d_grad_final, ui
   }
   const SlotPlan p = plan_slots(e->ro.T, e->ro.ckpt);
   int gslot = 0;
-  CN_HIP(record_backward(e, p, e->ro.T, e->ro.ckpt, 0, &gslot));
+  // Other calls may have used intermediate slot 0 since the forward: recompute every segment.
+  CN_HIP(record_backward(e, p, e->ro.T, e->ro.ckpt, 0, false, &gslot));
   CN_HIP(hipStreamSynchronize(e->stream));
   e->ro.grad_slot = gslot;
   return CN_OK;
