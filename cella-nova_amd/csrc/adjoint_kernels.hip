@@ -428,9 +428,12 @@ static hipError_t launch_adjoint_n(const StepLaunch& sl, int gin, int gout, hipS
   if ((e = rows(true)) != hipSuccess) return e;
   // The adjoint column kernel keeps a per-source accumulator beside the transform, so it runs
   // with 8 values per thread (16 would spill), on the plan's tile width or the widest valid one
-  // below it.
+  // below it, at most 4 columns (measured: wider tiles leave too few blocks, e.g. 192 blocks of
+  // 1024 threads at 1024^2 C3K9 with 8-column tiles; 4-column tiles were 2-3 % faster per
+  // rollout at 512^2 and 1024^2, 2-column tiles slower).
   int tc = sl.col_tc ? sl.col_tc : DefaultGeom<N>::TC;
   while (tc > 2 && tc * ColCfg<N, 8>::T > 1024) tc /= 2;
+  if (tc > 4) tc = 4;
   if ((e = launch_col_adj_tc<N, 8>(sl, tc, st)) != hipSuccess) return e;
   if ((e = rows(false)) != hipSuccess) return e;
   return hipGetLastError();
