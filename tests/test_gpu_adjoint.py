@@ -151,6 +151,39 @@ def test_param_grads_vs_fp64_finite_differences(cn):
         assert abs(g[name][idx] - fd) <= 1e-3 * abs(fd) + 1e-9
 
 
+def test_split_backward_unaffected_by_conv_readback(cn):
+    """cn_rollout_backward after cn_rollout_forward recomputes every segment's intermediates, so a
+    call that reuses the intermediate buffers in between (the conv readback) changes nothing; and
+    the split path gives the rollout graph's gradients (which reuse the forward's last segment)."""
+    cr, tg = _pair(cn, 64, 3, R=9, seed=43, patch=32)
+    target = tg["state"].reshape(-1).astype(np.float32)
+
+    def split(readback):
+        eng = cn.make_engine(cr)
+        eng.run_steps(2)  # current state in slot 0, its predecessor kept in slot 1
+        eng.rollout_forward(6, 4)
+        final = eng.creature_rollout_final(0)
+        if readback:
+            eng.download_conv_buffer()
+        gw = np.float32(2.0 / final.size) * (final - target)
+        eng.rollout_backward([torch.as_tensor(gw, device=DEV).contiguous()])
+        g = eng.creature_grads(0)
+        eng.close()
+        return g
+
+    a, b = split(False), split(True)
+    for k in ("state0", "mu", "sigma", "weights"):
+        assert np.array_equal(a[k], b[k]), k
+    eng = cn.make_engine(cr)
+    eng.run_steps(2)
+    eng.creature_set_target(0, tg["state"])
+    eng.rollout_grad(6, 4)
+    c = eng.creature_grads(0)
+    eng.close()
+    for k in ("state0", "mu", "sigma", "weights"):
+        assert _rel(a[k], c[k].astype(np.float64)) <= 1e-6, k
+
+
 def test_autograd_function_matches_internal_mse(cn):
     from cella_nova_amd.autograd import rollout
     cr, tg = _pair(cn, 64, 3, R=9, seed=41, patch=32)
