@@ -453,24 +453,32 @@ cn_status ensure_islots(cn_engine* e, int want) {
       sync = true;
     }
     const int n = std::max(want, c.islots);
-    for (float** buf : {&c.dev.A, &c.dev.u}) {
-      const size_t planes = (buf == &c.dev.A) ? c.C : c.K;
-      void* p = nullptr;
-      const cn_status st = dalloc(e, c.allocs, &p, (size_t)n * planes * c.plane() * 4);
-      if (st != CN_OK) return st;
-      if (*buf) {
-        auto it = std::find(c.allocs.begin(), c.allocs.end(), (void*)*buf);
-        if (it != c.allocs.end()) {
-          CN_HIP(hipFree(*it));
-          e->bytes -= (size_t)c.islots * planes * c.plane() * 4;
-          c.allocs.erase(it);
-        }
-      }
-      *buf = (float*)p;
+    const size_t a_bytes = (size_t)n * c.C * c.plane() * 4, u_bytes = (size_t)n * c.K * c.plane() * 4;
+    // Both new arrays first, then the descriptor, then the old arrays: a failure leaves the
+    // creature on its old, consistent buffers.
+    void *pa = nullptr, *pu = nullptr;
+    cn_status st = dalloc(e, c.allocs, &pa, a_bytes);
+    if (st != CN_OK) return st;
+    if ((st = dalloc(e, c.allocs, &pu, u_bytes)) != CN_OK) return st;
+    float* old[2] = {c.dev.A, c.dev.u};
+    const size_t old_bytes[2] = {(size_t)c.islots * c.C * c.plane() * 4, (size_t)c.islots * c.K * c.plane() * 4};
+    c.dev.A = (float*)pa;
+    c.dev.u = (float*)pu;
+    if ((st = upload_desc(e, (int)i)) != CN_OK) {
+      c.dev.A = old[0];
+      c.dev.u = old[1];
+      return st;
     }
     c.islots = n;
-    const cn_status s2 = upload_desc(e, (int)i);
-    if (s2 != CN_OK) return s2;
+    for (int j = 0; j < 2; ++j) {
+      if (!old[j]) continue;
+      auto it = std::find(c.allocs.begin(), c.allocs.end(), (void*)old[j]);
+      if (it != c.allocs.end()) {
+        CN_HIP(hipFree(*it));
+        e->bytes -= old_bytes[j];
+        c.allocs.erase(it);
+      }
+    }
   }
   return CN_OK;
 }
