@@ -392,16 +392,24 @@ cn_status ensure_slots(cn_engine* e, int want) {
     if (st != CN_OK) return st;
     CN_HIP(hipMemcpyAsync(p, c.dev.state, 2 * c.slot_elems() * 4, hipMemcpyDeviceToDevice, e->stream));
     CN_HIP(hipStreamSynchronize(e->stream));
-    auto it = std::find(c.allocs.begin(), c.allocs.end(), (void*)c.dev.state);
+    // New buffer into the descriptor first; the old one is freed only once the device copy
+    // of the descriptor no longer refers to it.
+    float* const old = c.dev.state;
+    const int old_n = c.dev.nslots;
+    c.dev.state = (float*)p;
+    c.dev.nslots = want;
+    const cn_status s2 = upload_desc(e, (int)i);
+    if (s2 != CN_OK) {
+      c.dev.state = old;
+      c.dev.nslots = old_n;
+      return s2;
+    }
+    auto it = std::find(c.allocs.begin(), c.allocs.end(), (void*)old);
     if (it != c.allocs.end()) {
       CN_HIP(hipFree(*it));
       e->bytes -= (size_t)e->nslots * c.slot_elems() * 4;
       c.allocs.erase(it);
     }
-    c.dev.state = (float*)p;
-    c.dev.nslots = want;
-    const cn_status s2 = upload_desc(e, (int)i);
-    if (s2 != CN_OK) return s2;
   }
   e->nslots = want;
   return CN_OK;
