@@ -192,6 +192,7 @@ def batched_grad(pkg, count=32, n=512, C=5, T=32, ckpt=4, iters=50, plain_iters=
         for j, i in enumerate(members):
             eng.load(j, crs[i])
             eng.creature_set_target(j, targets[i])
+        eng.autotune(3)  # every engine, batched or per creature, runs its own tuned plan
         return eng, members
 
     batch = [build(list(range(count)))]
@@ -271,6 +272,7 @@ def rollout_grad(pkg, n=1024, R=13, T=64, ckpt=8, iters=5, torch_iters=1, seed=0
     target = te.download_all_channels().reshape(3, n, n)
     te.close()
     eng = pkg.make_engine(cr)
+    eng.autotune(5)
     eng.creature_set_target(0, target)
     eng.rollout_grad(T, ckpt)  # captures the graph
     eng.sync()
@@ -316,7 +318,8 @@ def main():
     ap.add_argument("--torch-iters", type=int, default=2,
                     help="batched-grad: iterations of the PyTorch reference path (0: skip)")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--plan", default="", help="fixed launch plan fwd_e,inv_e,col_e,col_tc,diff_th (no autotune)")
+    ap.add_argument("--plan", default="",
+                    help="fixed launch plan fwd_e,inv_e,col_e,col_tc,diff_th[,col_split] (no autotune)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a device-to-device copy of this many MiB (PMC calibration only)")
     args = ap.parse_args()
@@ -349,8 +352,8 @@ def main():
     K = cr["num_kernels"]
     eng = pkg.make_engine(cr)
     if args.plan:
-        fe, ie, ce, tc, th = (int(v) for v in args.plan.split(","))
-        eng.set_plan(n, fe, ie, ce, tc, th)
+        vals = [int(v) for v in args.plan.split(",")]
+        eng.set_plan(n, *vals)
         plan = eng.plans()[0]
     else:
         plan = eng.plans()[0] if args.no_autotune else eng.autotune(9)[0]

@@ -62,7 +62,7 @@ class CreatureParams(ctypes.Structure):
 
 class Plan(ctypes.Structure):
     _fields_ = [("n", c_uint32), ("fwd_e", c_uint32), ("inv_e", c_uint32), ("col_e", c_uint32),
-                ("col_tc", c_uint32), ("diff_th", c_uint32)]
+                ("col_tc", c_uint32), ("diff_th", c_uint32), ("col_split", c_uint32)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/cellanova.h

@@ -208,6 +208,7 @@ struct SizeClass {
   CreatureDev* d_descs = nullptr;
   int max_c = 0, max_k = 0;
   int fwd_e = 0, inv_e = 0, col_e = 0, col_tc = 0, diff_th = 0;  // launch geometry (0: default)
+  bool col_split = false;
 };
 
 struct Rollout {
@@ -301,6 +302,7 @@ StepLaunch class_launch(cn_engine* e, const SizeClass& k, int sin, int sout) {
   sl.col_e = k.col_e;
   sl.col_tc = k.col_tc;
   sl.diff_th = k.diff_th;
+  sl.col_split = k.col_split;
   return sl;
 }
 
@@ -1290,6 +1292,7 @@ bool col_geom_ok(int N, int ce, int tc) {
 
 struct Geom {
   int fwd_e, inv_e, col_e, col_tc, diff_th;
+  bool col_split;
 };
 
 // Launches one step of size class k with geometry g and events around its kernels; returns
@@ -1305,6 +1308,7 @@ cn_status time_class_step(cn_engine* e, SizeClass& k, const Geom& g, float ms[4]
   sl.col_e = g.col_e;
   sl.col_tc = g.col_tc;
   sl.diff_th = g.diff_th;
+  sl.col_split = g.col_split;
   CN_HIP(launch_step(k.N, sl, e->stream));
   for (int q = 0; q < 5; ++q) sl.ev[q] = e->ev[q];
   CN_HIP(launch_step(k.N, sl, e->stream));
@@ -1321,18 +1325,25 @@ float median(std::vector<float> v) {
 cn_status tune_class(cn_engine* e, SizeClass& k, uint32_t reps) {
   const int N = k.N;
   const std::vector<int> rows = {8, 16};
-  std::vector<std::pair<int, int>> cols;
-  for (int ce : {8, 16})
-    for (int tc : {2, 4, 8, 16})
-      if (col_geom_ok(N, ce, tc)) cols.push_back({ce, tc});
+  struct ColGeom {
+    int e, tc;
+    bool split;
+  };
+  std::vector<ColGeom> cols;
+  for (bool split : {false, true}) {
+    if (split && k.max_k <= k.max_c) continue;  // one kernel per source: the same launch
+    for (int ce : {8, 16})
+      for (int tc : {2, 4, 8, 16})
+        if (col_geom_ok(N, ce, tc)) cols.push_back({ce, tc, split});
+  }
   const std::vector<int> diffs = {8, 16, 32};
   std::vector<std::vector<float>> tf(rows.size()), ti(rows.size()), tcol(cols.size()), td(diffs.size());
   const size_t rounds = std::max({rows.size(), cols.size(), diffs.size()});
   // one untimed pass per candidate set to load code objects and warm caches
   for (size_t r = 0; r < rounds * (reps + 1); ++r) {
     const size_t i = r % rounds;
-    const Geom g{rows[i % rows.size()], rows[(i + 1) % rows.size()], cols[i % cols.size()].first,
-                 cols[i % cols.size()].second, diffs[i % diffs.size()]};
+    const ColGeom& cg = cols[i % cols.size()];
+    const Geom g{rows[i % rows.size()], rows[(i + 1) % rows.size()], cg.e, cg.tc, diffs[i % diffs.size()], cg.split};
     float ms[4];
     const cn_status st = time_class_step(e, k, g, ms);
     if (st != CN_OK) return st;
@@ -1350,8 +1361,9 @@ cn_status tune_class(cn_engine* e, SizeClass& k, uint32_t reps) {
   };
   k.fwd_e = rows[best(tf)];
   k.inv_e = rows[best(ti)];
-  k.col_e = cols[best(tcol)].first;
-  k.col_tc = cols[best(tcol)].second;
+  k.col_e = cols[best(tcol)].e;
+  k.col_tc = cols[best(tcol)].tc;
+  k.col_split = cols[best(tcol)].split;
   k.diff_th = diffs[best(td)];
   return CN_OK;
 }
@@ -1389,6 +1401,7 @@ cn_status cn_engine_get_plan(const cn_engine* e, uint32_t size_class, cn_plan* o
   out->col_e = (uint32_t)k.col_e;
   out->col_tc = (uint32_t)k.col_tc;
   out->diff_th = (uint32_t)k.diff_th;
+  out->col_split = k.col_split ? 1u : 0u;
   return CN_OK;
 }
 
@@ -1404,6 +1417,7 @@ cn_status cn_engine_set_plan(cn_engine* e, const cn_plan* plan) {
     return set_error(CN_ERR_INVALID_ARG, "values per thread must be 0 (default), 8 or 16");
   if (plan->diff_th != 0 && plan->diff_th != 8 && plan->diff_th != 16 && plan->diff_th != 32)
     return set_error(CN_ERR_INVALID_ARG, "diffusion tile height must be 0, 8, 16 or 32");
+  if (plan->col_split > 1) return set_error(CN_ERR_INVALID_ARG, "col_split must be 0 or 1");
   if (plan->col_tc != 0 || plan->col_e != 0) {
     const int ce = plan->col_e ? (int)plan->col_e : 16;
     const int tc = plan->col_tc ? (int)plan->col_tc : 4;
@@ -1415,6 +1429,7 @@ cn_status cn_engine_set_plan(cn_engine* e, const cn_plan* plan) {
   k->col_e = (int)plan->col_e;
   k->col_tc = (int)plan->col_tc;
   k->diff_th = (int)plan->diff_th;
+  k->col_split = plan->col_split != 0;
   CN_HIP(hipSetDevice(e->device));
   CN_HIP(hipStreamSynchronize(e->stream));
   drop_graphs(e);

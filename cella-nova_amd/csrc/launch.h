@@ -23,6 +23,7 @@ struct StepLaunch {
   // Launch geometry (0: built-in default): values per thread of the forward (R2C) and
   // inverse (C2R) row FFTs and of the column FFTs, column tile width, diffusion tile height.
   int fwd_e = 0, inv_e = 0, col_e = 0, col_tc = 0, diff_th = 0;
+  bool col_split = false;              // column pass per (tile, kernel) instead of (tile, source)
   // Optional per-kernel timing events: ev[0] before the row pass, ev[1] after it, ev[2] after
   // the column pass, ev[3] after the inverse-row + growth pass, ev[4] after the diffusion.
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};

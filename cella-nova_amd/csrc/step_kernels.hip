@@ -59,18 +59,30 @@ k_row_fwd(const CreatureDev* __restrict__ descs, int sin, const float2* __restri
 
 // ---------------------------------------------------------------------------------------
 // Fused column pass: forward column FFT of one source tile, then per kernel of that source:
-// product with the kernel's half spectrum and inverse column FFT.
+// product with the kernel's half spectrum and inverse column FFT.  SPLIT: one block per
+// (tile, kernel) instead of per (tile, source), each redoing its source's forward transform —
+// K instead of C blocks per tile, for grids too small to fill the GPU by sources.
 // ---------------------------------------------------------------------------------------
-template <int N, int TC, int EC>
+template <int N, int TC, int EC, bool SPLIT>
 __global__ void __launch_bounds__((TC * ColCfg<N, EC>::T)) __attribute__((amdgpu_waves_per_eu(4)))
 k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) {
   constexpr int L = N, E = ColCfg<N, EC>::E, HALF = N / 2;
   const CreatureDev& d = descs[blockIdx.z];
   if (d.N != N) return;
-  const int c = blockIdx.y;
-  if (c >= d.C) return;
-  const int kb = d.src_off[c], ke = d.src_off[c + 1];
-  if (kb == ke) return;
+  int c, kb, ke;
+  if constexpr (SPLIT) {
+    if ((int)blockIdx.y >= d.K) return;
+    c = d.src_of[blockIdx.y];
+    kb = 0;
+    ke = 1;
+  } else {
+    c = blockIdx.y;
+    if (c >= d.C) return;
+    kb = d.src_off[c];
+    ke = d.src_off[c + 1];
+    if (kb == ke) return;
+  }
+  auto kernel_of = [&](int kk) { return SPLIT ? (int)blockIdx.y : d.src_k[kk]; };
   const int x0 = xcd_tile<(N / 2) / TC>(blockIdx.x) * TC;
   const int lane = threadIdx.x % TC;
   int t = threadIdx.x / TC;
@@ -101,7 +113,7 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
     }
   };
   float2 nk[E];
-  load_kernel(d.src_k[kb], nk);
+  load_kernel(kernel_of(kb), nk);
   __shared__ float2 lds[LdsMap<L, TC, true>::size];
   // Twiddles of both plans in LDS (see TwLds); the first exchange barrier orders the fill.
   using TF = TwLds<L, E, false>;
@@ -126,13 +138,13 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
 #pragma unroll
   for (int i = 0; i < E; ++i) w[i] = c_mul(v[i], nk[i]);
   for (int kk = kb; kk < ke; ++kk) {
-    const int k = d.src_k[kk];
+    const int k = kernel_of(kk);
     t = launder(t);
     if (kk > kb) {
 #pragma unroll
       for (int i = 0; i < E; ++i) w[i] = c_mul(v[i], nk[i]);
     }
-    if (kk + 1 < ke) load_kernel(d.src_k[kk + 1], nk);
+    if (kk + 1 < ke) load_kernel(kernel_of(kk + 1), nk);
     if (edge) {
       // Lane 0 splits the packed DC/Nyquist column; its kernel factors (khalf column 0, knyq)
       // are loaded a few at a time so that the loads overlap.
@@ -422,8 +434,13 @@ static hipError_t launch_col(const StepLaunch& sl, hipStream_t st) {
   if constexpr (!col_tc_ok<N, TC, EC>()) {
     return hipErrorInvalidValue;
   } else {
-    const dim3 g_col((N / 2) / TC, sl.max_c, sl.count);
-    k_col_fused<N, TC, EC><<<g_col, TC * ColCfg<N, EC>::T, 0, st>>>(sl.descs, sl.tw);
+    if (sl.col_split) {
+      const dim3 g_col((N / 2) / TC, sl.max_k, sl.count);
+      k_col_fused<N, TC, EC, true><<<g_col, TC * ColCfg<N, EC>::T, 0, st>>>(sl.descs, sl.tw);
+    } else {
+      const dim3 g_col((N / 2) / TC, sl.max_c, sl.count);
+      k_col_fused<N, TC, EC, false><<<g_col, TC * ColCfg<N, EC>::T, 0, st>>>(sl.descs, sl.tw);
+    }
     return hipSuccess;
   }
 }

@@ -205,8 +205,8 @@ class _Engine:
             out.append({f: getattr(p, f) for f, _ in Plan._fields_})
         return out
 
-    def set_plan(self, n, fwd_e=0, inv_e=0, col_e=0, col_tc=0, diff_th=0):
-        p = Plan(int(n), int(fwd_e), int(inv_e), int(col_e), int(col_tc), int(diff_th))
+    def set_plan(self, n, fwd_e=0, inv_e=0, col_e=0, col_tc=0, diff_th=0, col_split=0):
+        p = Plan(int(n), int(fwd_e), int(inv_e), int(col_e), int(col_tc), int(diff_th), int(col_split))
         check(lib().cn_engine_set_plan(self._h, ctypes.byref(p)))
 
     def profile_kernels(self, steps):

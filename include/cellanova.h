@@ -181,10 +181,13 @@ cn_status cn_creature_get_grads(cn_engine* e, uint32_t creature, float* grad_sta
 
 /* Geometry of one grid-size class: values per thread (8 or 16) of the forward row (R2C),
  * inverse row (C2R) and column FFTs, column tile width of the column passes (2..16), and the
- * diffusion tile height (8, 16, 32).  0 selects the built-in default. */
+ * diffusion tile height (8, 16, 32).  0 selects the built-in default.  col_split = 1 runs the
+ * forward column pass as one block per (tile, kernel) instead of per (tile, source channel)
+ * (more blocks for small grids; each block redoes its source's forward column transform). */
 typedef struct cn_plan {
   uint32_t n;
   uint32_t fwd_e, inv_e, col_e, col_tc, diff_th;
+  uint32_t col_split;
 } cn_plan;
 
 /* Times every candidate geometry of every size class on the device (reps timed launches per

@@ -28,6 +28,14 @@ def test_every_geometry_keeps_parity(cn, orc, n):
                     eng.close()
                     err = orc.rel_linf(got, want)
                     assert err <= 1e-5, (fe, ie, ce, tc, th, err)
+                    if th == 8:
+                        # per-(tile, kernel) column blocks do the same arithmetic per column
+                        eng = cn.make_engine(cr)
+                        eng.set_plan(n, fwd_e=fe, inv_e=ie, col_e=ce, col_tc=tc, diff_th=th, col_split=1)
+                        eng.iterate()
+                        split = eng.download_all_channels().reshape(cr["state"].shape)
+                        eng.close()
+                        assert np.array_equal(split, got), (fe, ie, ce, tc)
 
 
 def test_autotune_choice_is_valid_and_correct(cn, orc):
@@ -46,7 +54,7 @@ def test_autotune_choice_is_valid_and_correct(cn, orc):
 def test_invalid_plans_rejected(cn):
     cr = cn.baseline_creature(64, num_channels=2, radius=6, seed=1, patch=32)
     eng = cn.make_engine(cr)
-    for kw in (dict(fwd_e=4), dict(col_e=32), dict(diff_th=64), dict(col_e=8, col_tc=64)):
+    for kw in (dict(fwd_e=4), dict(col_e=32), dict(diff_th=64), dict(col_e=8, col_tc=64), dict(col_split=2)):
         with pytest.raises(cn.CellaNovaError):
             eng.set_plan(64, **kw)
     with pytest.raises(cn.CellaNovaError):
@@ -71,3 +79,24 @@ def test_batch_autotune_per_size_class(cn):
         assert np.array_equal(b.creature_download(i), e.download_all_channels())
         e.close()
     b.close()
+
+
+def test_split_column_pass_matches_fused_in_mixed_batch(cn):
+    """Per-(tile, kernel) column blocks give the per-source pass bit for bit in a batch of
+    creatures with different C and K, including kernels on the complex spectrum path."""
+    from test_gpu_step import shift_kernels
+    crs = [cn.baseline_creature(64, num_channels=C, radius=7, seed=10 + C, patch=32) for C in (2, 3, 4)]
+    shift_kernels(crs[1], [1, 4])
+
+    def run(split):
+        b = cn.CreatureBatch(crs)
+        for i, cr in enumerate(crs):
+            b.load(i, cr)
+        b.set_plan(64, col_e=16, col_tc=4, col_split=split)
+        b.run_steps(3)
+        out = [b.creature_download(i) for i in range(len(crs))]
+        b.close()
+        return out
+
+    for a, s in zip(run(0), run(1)):
+        assert np.array_equal(a, s)
