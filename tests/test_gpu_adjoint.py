@@ -85,6 +85,28 @@ def test_rollout_grad_with_shifted_kernels(cn):
         assert errs[k] <= 1e-4, (k, errs[k])
 
 
+@pytest.mark.parametrize("maps", ["c1k1", "ragged_c3k5", "c4k2"])
+def test_rollout_grad_ragged_maps(cn, maps):
+    """Adjoint with a source channel that feeds no kernel, destinations that receive none and
+    repeated (source, destination) pairs, against torch FP64 autograd."""
+    from test_gpu_step import MAPS
+    src, dst = MAPS[maps]
+    C = max(max(src), max(dst)) + 1
+    mk = lambda seed: cn.generate_creature(64, C, len(src), np.array(src), np.array(dst), 9, patch=32, seed=seed,
+                                           creature=3)
+    cr, tg = mk(31), mk(131)
+    tgt = tg["state"]
+    loss_t, gs_t, gm_t, gsg_t, gw_t = _torch_grads(cr, tgt, 4)
+    loss, g, _ = _engine_grads(cn, cr, tgt, 4, 2)
+    errs = {"loss": abs(loss - loss_t) / loss_t,
+            "state0": _rel(g["state0"].reshape(gs_t.shape), gs_t),
+            "mu": _rel(g["mu"], gm_t), "sigma": _rel(g["sigma"], gsg_t), "weights": _rel(g["weights"], gw_t)}
+    print(f"{maps}: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+    assert errs["loss"] <= 1e-5
+    for k in ("state0", "mu", "sigma", "weights"):
+        assert errs[k] <= 1e-4, (k, errs[k])
+
+
 def test_checkpoint_stride_does_not_change_gradients(cn):
     cr, tg = _pair(cn, 64, 3, R=10, seed=5, patch=32)
     ref = None

@@ -100,3 +100,23 @@ def test_split_column_pass_matches_fused_in_mixed_batch(cn):
 
     for a, s in zip(run(0), run(1)):
         assert np.array_equal(a, s)
+
+
+@pytest.mark.parametrize("maps", ["c1k1", "ragged_c3k5", "c4k2", "c2k4"])
+def test_split_column_pass_matches_fused_for_ragged_maps(cn, maps):
+    """Per-kernel column blocks against per-source ones for channel maps with unused sources,
+    destinations without kernels and several kernels per (source, destination) pair."""
+    from test_gpu_step import MAPS
+    src, dst = MAPS[maps]
+    C = max(max(src), max(dst)) + 1
+    cr = cn.generate_creature(64, C, len(src), np.array(src), np.array(dst), 9, patch=32, seed=7, creature=64)
+
+    def run(split):
+        eng = cn.make_engine(cr)
+        eng.set_plan(64, col_e=16, col_tc=4, col_split=split)
+        eng.run_steps(3)
+        out = eng.download_all_channels()
+        eng.close()
+        return out
+
+    assert np.array_equal(run(0), run(1))
