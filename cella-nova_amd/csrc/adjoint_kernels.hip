@@ -87,9 +87,12 @@ k_diffusion_adj(const CreatureDev* __restrict__ descs, int sin, int gin, int N, 
     const float sv = sr[j];
     const float hz2 = hsum3f(a2), ht2 = hsum3f(a2 * g2);
     const float z = hz0 + hz1 + hz2, t1 = ht0 + ht1 + ht2;
-    const float q = sv / z;
-    const float dz = -t1 * q / z;
-    if (store && j >= 1 && j <= TH && yq < N) dsd[(size_t)yq * N] = t1 / z;
+    // One reciprocal (within 1 ulp) instead of three IEEE divisions: the gradients' tolerance
+    // is set by the FP32 forward, not by the last bit of 1/Z.
+    const float rz = __builtin_amdgcn_rcpf(z);
+    const float q = sv * rz;
+    const float dz = -t1 * q * rz;
+    if (store && j >= 1 && j <= TH && yq < N) dsd[(size_t)yq * N] = t1 * rz;
     const float hq2 = hsum3f(q), hd2 = hsum3f(dz);
     if (j >= 2) {
       const int yo = yq - 1;
