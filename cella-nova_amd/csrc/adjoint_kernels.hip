@@ -138,9 +138,10 @@ k_growth_adj_row(const CreatureDev* __restrict__ descs, const float2* __restrict
     const float gu0 = wk * gg.x * e0 * (-d0 * is2), gu1 = wk * gg.y * e1 * (-d1 * is2);
     v[i] = make_float2(gu0, gu1);
     smu -= gu0 + gu1;
-    ssg -= (gu0 * d0 + gu1 * d1) / sg;
+    ssg -= gu0 * d0 + gu1 * d1;
     sw += (e0 - 1.0f) * gg.x + (e1 - 1.0f) * gg.y;
   }
+  ssg /= sg;  // dL/dsigma = -sum gu (u - mu) / sigma: one division per thread
   __shared__ float2 lds[LdsMap<L, ROWS, false>::size];
   __shared__ float red[3][NT];
   using TF = TwLds<L, E, false>;
