@@ -237,6 +237,7 @@ struct cn_engine {
   std::vector<void*> allocs;
   hipGraphExec_t step_graph[2] = {nullptr, nullptr};
   hipGraphExec_t render_graph[2] = {nullptr, nullptr};  // step + render epilogue
+  hipGraphExec_t multi_graph[2] = {nullptr, nullptr};   // kMultiSteps steps (run_steps)
   RingParams* gen_rings = nullptr;  // device generator parameters, kMaxKernels entries
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   Rollout ro;               // last rollout plan
@@ -255,7 +256,7 @@ cn_status dalloc(cn_engine* e, std::vector<void*>& owner, void** p, size_t bytes
 }
 
 void drop_graphs(cn_engine* e) {
-  for (auto* gs : {e->step_graph, e->render_graph})
+  for (auto* gs : {e->step_graph, e->render_graph, e->multi_graph})
     for (int j = 0; j < 2; ++j)
       if (gs[j]) {
         (void)hipGraphExecDestroy(gs[j]);
@@ -361,6 +362,8 @@ cn_status capture(cn_engine* e, hipGraphExec_t* out, F&& fn) {
   if (hip_failed(ie, "hipGraphInstantiate", &st)) return st;
   return CN_OK;
 }
+
+constexpr int kMultiSteps = 8;  // steps per run_steps graph (even)
 
 cn_status enqueue_step(cn_engine* e, bool render = false) {
   const int cur = e->cur;
@@ -909,6 +912,28 @@ cn_status cn_run_steps(cn_engine* e, uint64_t n) {
   clear_error();
   if (!e) return set_error(CN_ERR_INVALID_ARG, "null engine");
   CN_HIP(hipSetDevice(e->device));
+  // Runs of kMultiSteps steps replay one graph holding all of them (an even count, so the
+  // ping-pong parity is back where it started): one graph launch per run instead of one per
+  // step, which saves the launch boundary (about 4.5 us per 2048^2 step, 2.3 %).
+  while (n >= (uint64_t)kMultiSteps) {
+    const int cur = e->cur;
+    if (!e->multi_graph[cur]) {
+      const cn_status st = capture(e, &e->multi_graph[cur], [&] {
+        for (int j = 0; j < kMultiSteps; ++j) {
+          const int a = cur ^ (j & 1);
+          const hipError_t err = launch_all_steps(e, a, a ^ 1);
+          if (err != hipSuccess) return err;
+        }
+        return hipSuccess;
+      });
+      if (st != CN_OK) return st;
+    }
+    CN_HIP(hipGraphLaunch(e->multi_graph[cur], e->stream));
+    e->steps += kMultiSteps;
+    e->have_prev = true;
+    e->ro_fwd_done = false;
+    n -= kMultiSteps;
+  }
   for (uint64_t j = 0; j < n; ++j) {
     const cn_status st = enqueue_step(e);
     if (st != CN_OK) return st;

@@ -175,6 +175,25 @@ def test_graph_replay_equals_direct_launch(cn):
     b.close()
 
 
+def test_multi_step_graph_equals_single_steps(cn):
+    """run_steps replays graphs of several steps for long runs: the same states as single
+    steps, from either ping-pong parity, with eager work between replays."""
+    cr = cn.baseline_creature(128, radius=12, seed=5, patch=64)
+    a, b = cn.make_engine(cr), cn.make_engine(cr)
+    a.iterate()                      # odd parity before the multi-step runs
+    a.run_steps(21)
+    mid = a.download_all_channels()  # eager readback between replays
+    a.run_steps(16)
+    for _ in range(38):
+        b.iterate()
+        if b.info()["steps_done"] == 22:
+            assert np.array_equal(b.download_all_channels(), mid)
+    assert np.array_equal(a.download_all_channels(), b.download_all_channels())
+    assert a.info()["steps_done"] == 38
+    a.close()
+    b.close()
+
+
 def test_mass_conservation_256(cn):
     cr = cn.baseline_creature(256, radius=20, seed=6)
     eng = cn.make_engine(cr)
