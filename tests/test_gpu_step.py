@@ -175,11 +175,16 @@ def test_graph_replay_equals_direct_launch(cn):
     b.close()
 
 
-def test_multi_step_graph_equals_single_steps(cn):
+@pytest.mark.parametrize("n,fwd_e", [(128, 0), (1024, 8), (1024, 16)])
+def test_multi_step_graph_equals_single_steps(cn, n, fwd_e):
     """run_steps replays graphs of several steps for long runs: the same states as single
-    steps, from either ping-pong parity, with eager work between replays."""
-    cr = cn.baseline_creature(128, radius=12, seed=5, patch=64)
+    steps, from either ping-pong parity, with eager work between replays, for the default and
+    for tuned row geometries."""
+    cr = cn.baseline_creature(n, radius=max(4, n // 10), seed=5, patch=64)
     a, b = cn.make_engine(cr), cn.make_engine(cr)
+    if fwd_e:
+        a.set_plan(n, fwd_e=fwd_e)
+        b.set_plan(n, fwd_e=fwd_e)
     a.iterate()                      # odd parity before the multi-step runs
     a.run_steps(21)
     mid = a.download_all_channels()  # eager readback between replays
@@ -192,6 +197,29 @@ def test_multi_step_graph_equals_single_steps(cn):
     assert a.info()["steps_done"] == 38
     a.close()
     b.close()
+
+
+def test_multi_step_graph_mixed_batch(cn):
+    """A batch mixing grid sizes and channel counts (size classes 16, 64, 256): multi-step graphs
+    and single steps agree bit for bit."""
+    crs = [cn.baseline_creature(n, num_channels=C, radius=max(3, n // 10), seed=n + C, patch=min(64, n // 2))
+           for n, C in ((16, 2), (64, 3), (256, 2), (16, 3))]
+
+    def run(chained):
+        b = cn.CreatureBatch(crs)
+        for i, cr in enumerate(crs):
+            b.load(i, cr)
+        if chained:
+            b.run_steps(17)
+        else:
+            for _ in range(17):
+                b.iterate()
+        out = [b.creature_download(i) for i in range(len(crs))]
+        b.close()
+        return out
+
+    for a, s in zip(run(True), run(False)):
+        assert np.array_equal(a, s)
 
 
 def test_mass_conservation_256(cn):
