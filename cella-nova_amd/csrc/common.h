@@ -12,7 +12,19 @@ namespace cn {
 
 cn_status set_error(cn_status st, const char* fmt, ...);
 void clear_error();
-const std::string& last_error_text();
+const char* last_error_text();
+// Inside a catch block: records the exception in flight as a status (std::bad_alloc and
+// std::length_error as CN_ERR_OUT_OF_MEMORY) and returns it.
+cn_status status_of_exception() noexcept;
+
+// Wraps the body of every C ABI function that returns cn_status: no C++ exception crosses the
+// boundary.
+#define CN_TRY try {
+#define CN_CATCH                          \
+  }                                       \
+  catch (...) {                           \
+    return ::cn::status_of_exception();   \
+  }
 
 // Returns true (and records CN_ERR_HIP) if the call failed or a failure was injected.
 bool hip_failed(hipError_t err, const char* what, cn_status* st);

@@ -11,6 +11,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
+#include <stdexcept>
 #include <vector>
 #include "common.h"
 #include "engine.h"
@@ -22,20 +24,36 @@ namespace cn {
 // ---------------------------------------------------------------------------------------
 // errors
 // ---------------------------------------------------------------------------------------
-static thread_local std::string g_err;
+// A fixed per-thread buffer: recording an error never allocates, so it also works while
+// reporting a failed host allocation.
+static thread_local char g_err[1024];
 static int g_fail_next = 0;
 
 cn_status set_error(cn_status st, const char* fmt, ...) {
-  char buf[1024];
+  char buf[sizeof g_err];
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(buf, sizeof buf, fmt, ap);
   va_end(ap);
-  g_err = buf;
+  std::memcpy(g_err, buf, sizeof buf);
   return st;
 }
-void clear_error() { g_err.clear(); }
-const std::string& last_error_text() { return g_err; }
+void clear_error() { g_err[0] = 0; }
+const char* last_error_text() { return g_err; }
+
+cn_status status_of_exception() noexcept {
+  try {
+    throw;
+  } catch (const std::bad_alloc&) {
+    return set_error(CN_ERR_OUT_OF_MEMORY, "host allocation failed");
+  } catch (const std::length_error&) {
+    return set_error(CN_ERR_OUT_OF_MEMORY, "host allocation too large");
+  } catch (const std::exception& ex) {
+    return set_error(CN_ERR_STATE, "internal error: %s", ex.what());
+  } catch (...) {
+    return set_error(CN_ERR_STATE, "internal error");
+  }
+}
 
 bool inject_failure() {
   if (g_fail_next > 0) {
@@ -614,8 +632,9 @@ cn_status create_engine(const cn_engine_desc* descs, uint32_t count, int device,
         validate_creature(d.n, d.num_channels, d.num_kernels, d.src, d.dst, d.mu, d.sigma, d.weights, d.temp);
     if (st != CN_OK) {
       if (count > 1) {
-        const std::string msg = last_error_text();
-        set_error(st, "creature %u: %s", i, msg.c_str());
+        char msg[sizeof g_err];
+        std::memcpy(msg, last_error_text(), sizeof msg);
+        set_error(st, "creature %u: %s", i, msg);
       }
       return st;
     }
@@ -714,29 +733,36 @@ float* cur_state(cn_engine* e, uint32_t i) {
 
 extern "C" {
 
-const char* cn_last_error(void) { return g_err.c_str(); }
+const char* cn_last_error(void) { return g_err; }
 
 void cn_debug_fail_next_hip(int count) { g_fail_next = count > 0 ? count : 0; }
 
 cn_status cn_engine_create(const cn_engine_desc* desc, cn_engine** out) {
+  CN_TRY
   clear_error();
   if (!desc) return set_error(CN_ERR_INVALID_ARG, "null desc or out");
   return create_engine(desc, 1, desc->device, out);
+  CN_CATCH
 }
 
 cn_status cn_batch_create(const cn_engine_desc* descs, uint32_t count, int device, cn_engine** out) {
+  CN_TRY
   clear_error();
   return create_engine(descs, count, device, out);
+  CN_CATCH
 }
 
 cn_status cn_engine_destroy(cn_engine* e) {
+  CN_TRY
   clear_error();
   if (!e) return set_error(CN_ERR_INVALID_ARG, "null engine");
   release(e);
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_creature_set_kernel(cn_engine* e, uint32_t i, uint32_t k, const float* interleaved, size_t n_complex) {
+  CN_TRY
   clear_error();
   cn_status st = check_creature(e, i);
   if (st != CN_OK) return st;
@@ -765,13 +791,17 @@ cn_status cn_creature_set_kernel(cn_engine* e, uint32_t i, uint32_t k, const flo
     return upload_desc(e, (int)i);
   }
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_set_kernel(cn_engine* e, uint32_t k, const float* interleaved, size_t n_complex) {
+  CN_TRY
   return cn_creature_set_kernel(e, 0, k, interleaved, n_complex);
+  CN_CATCH
 }
 
 cn_status cn_creature_upload_channel_f32(cn_engine* e, uint32_t i, uint32_t channel, const float* data, size_t n) {
+  CN_TRY
   clear_error();
   cn_status st = check_creature(e, i);
   if (st != CN_OK) return st;
@@ -787,27 +817,36 @@ cn_status cn_creature_upload_channel_f32(cn_engine* e, uint32_t i, uint32_t chan
   e->have_prev = false;
   e->ro_fwd_done = false;
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_upload_channel_f32(cn_engine* e, uint32_t channel, const float* data, size_t n) {
+  CN_TRY
   return cn_creature_upload_channel_f32(e, 0, channel, data, n);
+  CN_CATCH
 }
 
 cn_status cn_upload_channel_f64(cn_engine* e, uint32_t channel, const double* data, size_t n) {
+  CN_TRY
   clear_error();
   cn_status st = check_creature(e, 0);
   if (st != CN_OK) return st;
   if (!data) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  const Creature& c = e->cr[0];
+  if (channel >= c.C) return set_error(CN_ERR_INVALID_ARG, "channel %u out of range (C = %u)", channel, c.C);
+  if (n != c.plane()) return set_error(CN_ERR_INVALID_ARG, "channel has %zu values, expected %zu", n, c.plane());
   std::vector<float> f(n);
   for (size_t j = 0; j < n; ++j) {
     if (!std::isfinite(data[j])) return set_error(CN_ERR_INVALID_ARG, "channel value %zu not finite", j);
     f[j] = (float)data[j];  // mod.rs:220
   }
   return cn_creature_upload_channel_f32(e, 0, channel, f.data(), n);
+  CN_CATCH
 }
 
 cn_status cn_creature_set_growth(cn_engine* e, uint32_t i, const float* mu, const float* sigma, const float* w,
                                  size_t K) {
+  CN_TRY
   clear_error();
   cn_status st = check_creature(e, i);
   if (st != CN_OK) return st;
@@ -817,12 +856,14 @@ cn_status cn_creature_set_growth(cn_engine* e, uint32_t i, const float* mu, cons
   fill_growth(c.dev, c.K, mu, sigma, w);
   CN_HIP(hipSetDevice(e->device));
   return upload_desc(e, (int)i);
+  CN_CATCH
 }
 
 // ---------------------------------------------------------------------------------------
 // Device creature generator and spectrum readback
 // ---------------------------------------------------------------------------------------
 cn_status cn_creature_generate(cn_engine* e, uint32_t i, const cn_creature_params* p) {
+  CN_TRY
   clear_error();
   cn_status st = check_creature(e, i);
   if (st != CN_OK) return st;
@@ -859,9 +900,11 @@ cn_status cn_creature_generate(cn_engine* e, uint32_t i, const cn_creature_param
   e->have_prev = false;
   e->ro_fwd_done = false;
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_creature_download_kernels(cn_engine* e, uint32_t i, float* out, size_t n_floats) {
+  CN_TRY
   clear_error();
   cn_status st = check_creature(e, i);
   if (st != CN_OK) return st;
@@ -899,16 +942,20 @@ cn_status cn_creature_download_kernels(cn_engine* e, uint32_t i, float* out, siz
       }
   }
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_iterate(cn_engine* e) {
+  CN_TRY
   clear_error();
   if (!e) return set_error(CN_ERR_INVALID_ARG, "null engine");
   CN_HIP(hipSetDevice(e->device));
   return enqueue_step(e);
+  CN_CATCH
 }
 
 cn_status cn_run_steps(cn_engine* e, uint64_t n) {
+  CN_TRY
   clear_error();
   if (!e) return set_error(CN_ERR_INVALID_ARG, "null engine");
   CN_HIP(hipSetDevice(e->device));
@@ -939,14 +986,17 @@ cn_status cn_run_steps(cn_engine* e, uint64_t n) {
     if (st != CN_OK) return st;
   }
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_sync(cn_engine* e) {
+  CN_TRY
   clear_error();
   if (!e) return set_error(CN_ERR_INVALID_ARG, "null engine");
   CN_HIP(hipSetDevice(e->device));
   CN_HIP(hipStreamSynchronize(e->stream));
   return CN_OK;
+  CN_CATCH
 }
 
 // Allocates every creature's packed frame (zeroed, so byte 3 of each pixel stays 0).
@@ -965,15 +1015,18 @@ static cn_status ensure_render(cn_engine* e) {
 }
 
 cn_status cn_iterate_and_render(cn_engine* e) {
+  CN_TRY
   clear_error();
   if (!e) return set_error(CN_ERR_INVALID_ARG, "null engine");
   CN_HIP(hipSetDevice(e->device));
   const cn_status st = ensure_render(e);
   if (st != CN_OK) return st;
   return enqueue_step(e, true);
+  CN_CATCH
 }
 
 cn_status cn_creature_download_render(cn_engine* e, uint32_t i, uint8_t* rgb, size_t n) {
+  CN_TRY
   clear_error();
   cn_status st = check_creature(e, i);
   if (st != CN_OK) return st;
@@ -995,10 +1048,13 @@ cn_status cn_creature_download_render(cn_engine* e, uint32_t i, uint8_t* rgb, si
     rgb[3 * j + 2] = (uint8_t)packed[j];
   }
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_download_render(cn_engine* e, uint8_t* rgb, size_t n) {
+  CN_TRY
   return cn_creature_download_render(e, 0, rgb, n);
+  CN_CATCH
 }
 
 // One directly launched step of every creature with events around every kernel of the first
@@ -1022,6 +1078,7 @@ static cn_status timed_step(cn_engine* e, float kernel_ms[4]) {
 }
 
 cn_status cn_timed_iterate(cn_engine* e, double ms[4]) {
+  CN_TRY
   clear_error();
   if (!e || !ms) return set_error(CN_ERR_INVALID_ARG, "null argument");
   CN_HIP(hipSetDevice(e->device));
@@ -1033,9 +1090,11 @@ cn_status cn_timed_iterate(cn_engine* e, double ms[4]) {
   ms[2] = k[3];                         // diffusion
   ms[3] = 0.0;                          // copy: ping-pong pointer swap
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_profile_kernels(cn_engine* e, uint64_t steps, double* kernel_ms, size_t n) {
+  CN_TRY
   clear_error();
   if (!e || !kernel_ms) return set_error(CN_ERR_INVALID_ARG, "null argument");
   if (n != 4) return set_error(CN_ERR_INVALID_ARG, "kernel_ms must have 4 entries");
@@ -1050,9 +1109,11 @@ cn_status cn_profile_kernels(cn_engine* e, uint64_t steps, double* kernel_ms, si
   }
   for (int q = 0; q < 4; ++q) kernel_ms[q] = acc[q] / (double)steps;
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_creature_download_channels(cn_engine* e, uint32_t i, float* out, size_t n) {
+  CN_TRY
   clear_error();
   cn_status st = check_creature(e, i);
   if (st != CN_OK) return st;
@@ -1063,13 +1124,17 @@ cn_status cn_creature_download_channels(cn_engine* e, uint32_t i, float* out, si
   CN_HIP(hipMemcpyAsync(out, cur_state(e, i), tot * 4, hipMemcpyDeviceToHost, e->stream));
   CN_HIP(hipStreamSynchronize(e->stream));
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_download_channels(cn_engine* e, float* out, size_t n) {
+  CN_TRY
   return cn_creature_download_channels(e, 0, out, n);
+  CN_CATCH
 }
 
 cn_status cn_download_conv_real(cn_engine* e, float* out, size_t n) {
+  CN_TRY
   clear_error();
   cn_status st = check_creature(e, 0);
   if (st != CN_OK) return st;
@@ -1103,9 +1168,11 @@ cn_status cn_download_conv_real(cn_engine* e, float* out, size_t n) {
   const float scale = (float)P;  // unnormalised like the reference's conv buffer (exact: power of 2)
   for (size_t j = 0; j < tot; ++j) out[j] *= scale;
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_creature_mass_fp64(cn_engine* e, uint32_t i, double* out, size_t n_channels) {
+  CN_TRY
   clear_error();
   cn_status st = check_creature(e, i);
   if (st != CN_OK) return st;
@@ -1132,13 +1199,17 @@ cn_status cn_creature_mass_fp64(cn_engine* e, uint32_t i, double* out, size_t n_
     out[ch] = s + comp;
   }
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_mass_fp64(cn_engine* e, double* out, size_t n_channels) {
+  CN_TRY
   return cn_creature_mass_fp64(e, 0, out, n_channels);
+  CN_CATCH
 }
 
 cn_status cn_engine_get_info(const cn_engine* e, cn_engine_info* out) {
+  CN_TRY
   clear_error();
   if (!e || !out) return set_error(CN_ERR_INVALID_ARG, "null argument");
   out->n = e->cr[0].N;
@@ -1149,12 +1220,14 @@ cn_status cn_engine_get_info(const cn_engine* e, cn_engine_info* out) {
   out->compact_kernels = 0;
   for (uint32_t k = 0; k < e->cr[0].K; ++k) out->compact_kernels += e->cr[0].dev.ksym[k] ? 1u : 0u;
   return CN_OK;
+  CN_CATCH
 }
 
 // ---------------------------------------------------------------------------------------
 // state <-> device pointers (caller-owned device memory; both calls synchronise)
 // ---------------------------------------------------------------------------------------
 cn_status cn_creature_state_from_device(cn_engine* e, uint32_t i, const float* d_src, size_t n) {
+  CN_TRY
   clear_error();
   cn_status st = check_creature(e, i);
   if (st != CN_OK) return st;
@@ -1166,9 +1239,11 @@ cn_status cn_creature_state_from_device(cn_engine* e, uint32_t i, const float* d
   e->have_prev = false;
   e->ro_fwd_done = false;
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_creature_state_to_device(cn_engine* e, uint32_t i, float* d_dst, size_t n) {
+  CN_TRY
   clear_error();
   cn_status st = check_creature(e, i);
   if (st != CN_OK) return st;
@@ -1178,12 +1253,14 @@ cn_status cn_creature_state_to_device(cn_engine* e, uint32_t i, float* d_dst, si
   CN_HIP(hipMemcpyAsync(d_dst, cur_state(e, i), n * 4, hipMemcpyDeviceToDevice, e->stream));
   CN_HIP(hipStreamSynchronize(e->stream));
   return CN_OK;
+  CN_CATCH
 }
 
 // ---------------------------------------------------------------------------------------
 // rollouts
 // ---------------------------------------------------------------------------------------
 cn_status cn_creature_set_target(cn_engine* e, uint32_t i, const float* target, size_t n) {
+  CN_TRY
   clear_error();
   cn_status st = check_creature(e, i);
   if (st != CN_OK) return st;
@@ -1198,9 +1275,11 @@ cn_status cn_creature_set_target(cn_engine* e, uint32_t i, const float* target, 
   CN_HIP(hipStreamSynchronize(e->stream));
   c.has_target = true;
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_rollout_grad(cn_engine* e, uint32_t steps, uint32_t ckpt_every) {
+  CN_TRY
   clear_error();
   if (!e) return set_error(CN_ERR_INVALID_ARG, "null engine");
   for (size_t i = 0; i < e->cr.size(); ++i)
@@ -1213,9 +1292,11 @@ cn_status cn_rollout_grad(cn_engine* e, uint32_t steps, uint32_t ckpt_every) {
   CN_HIP(hipGraphLaunch(e->ro.graph, e->stream));
   e->ro_fwd_done = true;
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_rollout_forward(cn_engine* e, uint32_t steps, uint32_t ckpt_every) {
+  CN_TRY
   clear_error();
   if (!e) return set_error(CN_ERR_INVALID_ARG, "null engine");
   CN_HIP(hipSetDevice(e->device));
@@ -1225,9 +1306,11 @@ cn_status cn_rollout_forward(cn_engine* e, uint32_t steps, uint32_t ckpt_every) 
   CN_HIP(hipGraphLaunch(e->ro.fwd, e->stream));
   e->ro_fwd_done = true;
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_creature_rollout_final(cn_engine* e, uint32_t i, float* out, size_t n, int device_pointer) {
+  CN_TRY
   clear_error();
   cn_status st = check_creature(e, i);
   if (st != CN_OK) return st;
@@ -1239,9 +1322,11 @@ cn_status cn_creature_rollout_final(cn_engine* e, uint32_t i, float* out, size_t
                         device_pointer ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, e->stream));
   CN_HIP(hipStreamSynchronize(e->stream));
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_rollout_backward(cn_engine* e, const float* const* d_grad_final, uint32_t count) {
+  CN_TRY
   clear_error();
   if (!e || !d_grad_final) return set_error(CN_ERR_INVALID_ARG, "null argument");
   if (count != e->cr.size()) return set_error(CN_ERR_INVALID_ARG, "expected %zu gradient pointers", e->cr.size());
@@ -1260,9 +1345,11 @@ cn_status cn_rollout_backward(cn_engine* e, const float* const* d_grad_final, ui
   CN_HIP(hipStreamSynchronize(e->stream));
   e->ro.grad_slot = gslot;
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_creature_get_loss(cn_engine* e, uint32_t i, double* loss) {
+  CN_TRY
   clear_error();
   cn_status st = check_creature(e, i);
   if (st != CN_OK) return st;
@@ -1277,10 +1364,12 @@ cn_status cn_creature_get_loss(cn_engine* e, uint32_t i, double* loss) {
   for (float v : part) s += v;
   *loss = s / (double)c.slot_elems();
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_creature_get_grads(cn_engine* e, uint32_t i, float* grad_state0, size_t n, float* grad_mu,
                                 float* grad_sigma, float* grad_w, size_t K, int device_pointers) {
+  CN_TRY
   clear_error();
   cn_status st = check_creature(e, i);
   if (st != CN_OK) return st;
@@ -1299,6 +1388,7 @@ cn_status cn_creature_get_grads(cn_engine* e, uint32_t i, float* grad_state0, si
     if (outs[j]) CN_HIP(hipMemcpyAsync(outs[j], c.dev.gparam + (size_t)j * c.K, c.K * 4, kind, e->stream));
   CN_HIP(hipStreamSynchronize(e->stream));
   return CN_OK;
+  CN_CATCH
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1398,6 +1488,7 @@ cn_status tune_class(cn_engine* e, SizeClass& k, uint32_t reps) {
 extern "C" {
 
 cn_status cn_autotune(cn_engine* e, uint32_t reps) {
+  CN_TRY
   clear_error();
   if (!e) return set_error(CN_ERR_INVALID_ARG, "null engine");
   if (reps == 0 || reps > 1000) return set_error(CN_ERR_INVALID_ARG, "reps must be in [1, 1000]");
@@ -1411,11 +1502,13 @@ cn_status cn_autotune(cn_engine* e, uint32_t reps) {
   e->have_prev = false;  // the other slot was used as scratch
   e->ro_fwd_done = false;
   return CN_OK;
+  CN_CATCH
 }
 
 uint32_t cn_engine_size_classes(const cn_engine* e) { return e ? (uint32_t)e->cls.size() : 0; }
 
 cn_status cn_engine_get_plan(const cn_engine* e, uint32_t size_class, cn_plan* out) {
+  CN_TRY
   clear_error();
   if (!e || !out) return set_error(CN_ERR_INVALID_ARG, "null argument");
   if (size_class >= e->cls.size()) return set_error(CN_ERR_INVALID_ARG, "size class %u out of range", size_class);
@@ -1428,9 +1521,11 @@ cn_status cn_engine_get_plan(const cn_engine* e, uint32_t size_class, cn_plan* o
   out->diff_th = (uint32_t)k.diff_th;
   out->col_split = k.col_split ? 1u : 0u;
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_engine_set_plan(cn_engine* e, const cn_plan* plan) {
+  CN_TRY
   clear_error();
   if (!e || !plan) return set_error(CN_ERR_INVALID_ARG, "null argument");
   SizeClass* k = nullptr;
@@ -1459,6 +1554,7 @@ cn_status cn_engine_set_plan(cn_engine* e, const cn_plan* plan) {
   CN_HIP(hipStreamSynchronize(e->stream));
   drop_graphs(e);
   return CN_OK;
+  CN_CATCH
 }
 
 }  // extern "C"

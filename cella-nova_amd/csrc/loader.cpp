@@ -226,6 +226,7 @@ void cn_free_config(cn_config* cfg) {
 }
 
 cn_status cn_load_config(const char* path, cn_config** out) {
+  CN_TRY
   clear_error();
   if (!path || !out) return set_error(CN_ERR_INVALID_ARG, "null argument");
   *out = nullptr;
@@ -319,9 +320,11 @@ cn_status cn_load_config(const char* path, cn_config** out) {
   }
   *out = cfg;
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_load_kernels(const char* path, uint32_t num_kernels, uint32_t target, float* out, size_t n_floats) {
+  CN_TRY
   clear_error();
   if (!path || !out) return set_error(CN_ERR_INVALID_ARG, "null argument");
   if (num_kernels == 0 || target == 0) return set_error(CN_ERR_INVALID_ARG, "num_kernels and target must be > 0");
@@ -373,9 +376,11 @@ cn_status cn_load_kernels(const char* path, uint32_t num_kernels, uint32_t targe
       }
   }
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_write_kernels(const char* path, const float* kernels_fft, uint32_t num_kernels, uint32_t n) {
+  CN_TRY
   clear_error();
   if (!path || !kernels_fft) return set_error(CN_ERR_INVALID_ARG, "null argument");
   if (num_kernels == 0 || n == 0) return set_error(CN_ERR_INVALID_ARG, "num_kernels and n must be > 0");
@@ -386,9 +391,11 @@ cn_status cn_write_kernels(const char* path, const float* kernels_fft, uint32_t 
   const bool closed = std::fclose(f) == 0;
   if (wrote != count || !closed) return set_error(CN_ERR_IO, "write to '%s' failed", path);
   return CN_OK;
+  CN_CATCH
 }
 
 cn_status cn_write_config(const char* path, const cn_config* cfg) {
+  CN_TRY
   clear_error();
   if (!path || !cfg) return set_error(CN_ERR_INVALID_ARG, "null argument");
   const size_t C = cfg->num_channels, K = cfg->num_kernels, P = (size_t)cfg->seed_size * cfg->seed_size;
@@ -454,6 +461,7 @@ cn_status cn_write_config(const char* path, const cn_config* cfg) {
   const bool closed = std::fclose(f) == 0;
   if (wrote != o.size() || !closed) return set_error(CN_ERR_IO, "write to '%s' failed", path);
   return CN_OK;
+  CN_CATCH
 }
 
 }  // extern "C"

@@ -245,6 +245,20 @@ def test_upload_download_roundtrip(cn):
     eng.close()
 
 
+def test_upload_f64_checks_sizes_before_reading(cn):
+    """cn_upload_channel_f64 rejects a length or channel that does not match the creature
+    before it allocates or reads: a huge length is an argument error, not a host allocation."""
+    cr = cn.baseline_creature(32, radius=4, seed=8, patch=16)
+    eng = cn.make_engine(cr)
+    buf = np.zeros(32 * 32, np.float64)
+    p = buf.ctypes.data
+    assert cn.lib().cn_upload_channel_f64(eng._h, 0, p, 1 << 62) == 1  # CN_ERR_INVALID_ARG
+    assert b"expected 1024" in cn.lib().cn_last_error()
+    assert cn.lib().cn_upload_channel_f64(eng._h, 3, p, 32 * 32) == 1
+    assert cn.lib().cn_upload_channel_f64(eng._h, 0, p, 32 * 32) == 0
+    eng.close()
+
+
 def test_drop_in_files_json_and_bin(cn, orc, tmp_path):
     """Reference file formats end to end: seed JSON (c0/c1 swapped at load) and a smaller
     kernel .bin spectrally padded to the grid (config.rs:32-125)."""

@@ -165,3 +165,32 @@ def test_writers_reject_bad_arguments(cn, tmp_path):
                        growth_sigma=np.ones(1, np.float32), growth_weights=np.ones(1, np.float32))
     with pytest.raises(cn.CellaNovaError, match="not finite"):
         cn.write_config(tmp_path / "c.json", cfg)
+
+
+_OOM_CHILD = r"""
+import ctypes, resource, sys
+lib = ctypes.CDLL(sys.argv[1])
+lib.cn_last_error.restype = ctypes.c_char_p
+vm = [int(l.split()[1]) for l in open("/proc/self/status") if l.startswith("VmSize:")][0] * 1024
+resource.setrlimit(resource.RLIMIT_AS, (vm + (512 << 20), resource.getrlimit(resource.RLIMIT_AS)[1]))
+out = ctypes.c_void_p()
+st = lib.cn_load_config(sys.argv[2].encode(), ctypes.byref(out))
+print(st, lib.cn_last_error().decode(), out.value)
+"""
+
+
+def test_load_config_host_allocation_failure_is_a_status(cn, tmp_path):
+    """A config larger than the host can hold: cn_load_config returns CN_ERR_OUT_OF_MEMORY
+    instead of letting std::bad_alloc cross the C ABI.  A sparse 1 TiB file read in a child
+    process whose address space is capped 512 MiB above its size at the call."""
+    import subprocess
+    import sys
+    big = tmp_path / "huge.json"
+    try:
+        with open(big, "wb") as f:
+            f.truncate(1 << 40)
+    except OSError:
+        pytest.skip("no sparse files on this filesystem")
+    r = subprocess.run([sys.executable, "-c", _OOM_CHILD, cn.LIB_PATH, str(big)], capture_output=True, timeout=300)
+    assert r.returncode == 0, r.stderr.decode()[-2000:]
+    assert r.stdout.decode().split() == ["6", "host", "allocation", "failed", "None"]  # CN_ERR_OUT_OF_MEMORY
