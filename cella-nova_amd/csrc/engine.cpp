@@ -1156,13 +1156,15 @@ cn_status cn_download_conv_real(cn_engine* e, float* out, size_t n) {
     c.dev.u = (float*)p;
     if ((st = upload_desc(e, 0)) != CN_OK) return st;
   }
-  // Recompute the convolution of the state the most recent step started from.
-  for (const SizeClass& k : e->cls) {
-    StepLaunch sl = class_launch(e, k, e->cur ^ 1, e->cur ^ 1);
-    sl.write_u = true;
-    sl.skip_diffusion = true;
-    CN_HIP(launch_step(k.N, sl, e->stream));
-  }
+  // Recompute the convolution of the state the most recent step started from, for creature 0
+  // alone: in a batch the other creatures may have no u buffer.
+  const SizeClass& k = e->cls[c.cls];
+  StepLaunch sl = class_launch(e, k, e->cur ^ 1, e->cur ^ 1);
+  sl.descs = k.d_descs + c.idx;
+  sl.count = 1;
+  sl.write_u = true;
+  sl.skip_diffusion = true;
+  CN_HIP(launch_step(k.N, sl, e->stream));
   CN_HIP(hipMemcpyAsync(out, c.dev.u, tot * 4, hipMemcpyDeviceToHost, e->stream));
   CN_HIP(hipStreamSynchronize(e->stream));
   const float scale = (float)P;  // unnormalised like the reference's conv buffer (exact: power of 2)

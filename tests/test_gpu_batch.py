@@ -73,3 +73,24 @@ def test_batch_set_growth_changes_only_its_creature(cn):
     e0.close()
     e1.close()
     b.close()
+
+
+def test_batch_conv_readback_is_creature_0(cn):
+    """cn_download_conv_real on a batch recomputes creature 0's convolution alone (the other
+    creatures of its size class have no convolution buffer): the same values as a single engine
+    of creature 0, and the other creatures' states are untouched."""
+    crs = [cn.baseline_creature(64, num_channels=3, radius=8, seed=s, patch=32) for s in (11, 12, 13)]
+    b = cn.CreatureBatch(crs)
+    for i, cr in enumerate(crs):
+        b.load(i, cr)
+    b.iterate()
+    before = [b.creature_download(i) for i in range(3)]
+    got = np.empty(9 * 64 * 64, np.float32)
+    assert cn.lib().cn_download_conv_real(b._h, got.ctypes.data, got.size) == 0, cn.lib().cn_last_error()
+    e = cn.make_engine(crs[0])
+    e.iterate()
+    assert np.array_equal(got, e.download_conv_buffer())
+    e.close()
+    for i in range(3):
+        assert np.array_equal(b.creature_download(i), before[i])
+    b.close()
