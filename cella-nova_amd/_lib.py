@@ -95,6 +95,7 @@ SIGNATURES = {
     "cn_generate_creature": (c_int, [POINTER(CreatureParams), c_void_p, c_void_p, c_void_p, c_void_p,
                                      c_void_p, c_void_p]),
     "cn_debug_fail_next_hip": (None, [c_int]),
+    "cn_debug_fail_hip_at": (None, [c_int]),
     # plans / autotuner
     "cn_autotune": (c_int, [c_void_p, c_uint32]),
     "cn_engine_size_classes": (c_uint32, [c_void_p]),

@@ -28,6 +28,7 @@ namespace cn {
 // reporting a failed host allocation.
 static thread_local char g_err[1024];
 static int g_fail_next = 0;
+static int g_fail_at = -1;  // checked calls to let through before one injected failure
 
 cn_status set_error(cn_status st, const char* fmt, ...) {
   char buf[sizeof g_err];
@@ -60,6 +61,7 @@ bool inject_failure() {
     --g_fail_next;
     return true;
   }
+  if (g_fail_at >= 0) return g_fail_at-- == 0;
   return false;
 }
 
@@ -736,6 +738,8 @@ extern "C" {
 const char* cn_last_error(void) { return g_err; }
 
 void cn_debug_fail_next_hip(int count) { g_fail_next = count > 0 ? count : 0; }
+
+void cn_debug_fail_hip_at(int skip) { g_fail_at = skip >= 0 ? skip : -1; }
 
 cn_status cn_engine_create(const cn_engine_desc* desc, cn_engine** out) {
   CN_TRY

@@ -286,6 +286,11 @@ cn_status cn_creature_download_kernels(cn_engine* e, uint32_t i, float* out, siz
  * without calling HIP (exercises error paths on machines without a GPU). */
 void cn_debug_fail_next_hip(int count);
 
+/* Let `skip` checked HIP calls / device allocations through, then make the next one report
+ * failure (once) without calling HIP; a negative value cancels.  Walking `skip` over 0, 1, 2, ...
+ * fails each step of a call in turn (error paths of the allocation and capture sequences). */
+void cn_debug_fail_hip_at(int skip);
+
 #ifdef __cplusplus
 }
 #endif
