@@ -245,3 +245,17 @@ def test_differentiable_rollout_1024_c3k9_t64(cn):
     assert errs["loss"] <= 1e-4
     for k in ("seed_patch", "mu", "sigma", "weights"):
         assert errs[k] <= 1e-3, (k, errs[k])
+
+
+def test_rollout_grad_maximum_channels_and_kernels(cn):
+    """C = 16, K = 256 (all pairs): rollout gradients against torch FP64 autograd."""
+    cr, tg = _pair(cn, 32, 16, R=5, seed=41, patch=16)
+    loss_t, gs_t, gm_t, gsg_t, gw_t = _torch_grads(cr, tg["state"], 3)
+    loss, g, _ = _engine_grads(cn, cr, tg["state"], 3, 2)
+    errs = {"loss": abs(loss - loss_t) / loss_t,
+            "state0": _rel(g["state0"].reshape(gs_t.shape), gs_t),
+            "mu": _rel(g["mu"], gm_t), "sigma": _rel(g["sigma"], gsg_t), "weights": _rel(g["weights"], gw_t)}
+    print("32^2 C16K256 T=3 ckpt=2: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+    assert errs["loss"] <= 1e-5
+    for k in ("state0", "mu", "sigma", "weights"):
+        assert errs[k] <= 1e-4, (k, errs[k])

@@ -319,3 +319,19 @@ def test_full_size_mass_2048(cn):
     print(f"2048^2 C3K9 300 steps: FP64 mass drift {drift:.3e}")
     assert drift <= 1e-6
     eng.close()
+
+
+def test_maximum_channels_and_kernels(cn, orc):
+    """The largest creature the ABI accepts: C = 16 channels (the reference growth shader's
+    array<f32, 16>) and K = 256 kernels (every source x destination pair), against the FP64
+    oracle; mass conserved."""
+    cr = cn.baseline_creature(32, num_channels=16, radius=5, seed=40, patch=16)
+    assert cr["num_kernels"] == 256
+    errs = _teacher_forced(cn, orc, cr, steps=2)
+    print(f"32^2 C16K256: rel L-inf per step {['%.2e' % e for e in errs]}")
+    assert max(errs) <= TOL
+    eng = cn.make_engine(cr)
+    m0 = eng.mass_fp64()
+    eng.run_steps(10)
+    assert np.abs(eng.mass_fp64() - m0).max() <= 1e-6 * np.abs(m0).max()
+    eng.close()
