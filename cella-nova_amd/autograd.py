@@ -4,7 +4,9 @@ that model is differentiated by autograd through torch.fft and F.unfold, this Fu
 hand-written forward and adjoint HIP kernels and keeps only a checkpoint every `ckpt_every`
 steps, recomputing each segment during the backward pass.
 
-One engine holds one outstanding rollout: call backward before the next forward on it.
+One engine holds one outstanding rollout: backward must come before the next forward on the
+same engine (a backward whose forward was superseded raises instead of returning the newer
+rollout's gradients).  Parameter gradients come back on the device and dtype of the parameters.
 """
 import torch
 
@@ -28,13 +30,19 @@ class DiffusionLeniaRollout(torch.autograd.Function):
         engine.rollout_forward(steps, ckpt_every)
         out = torch.empty_like(s0)
         engine.creature_rollout_final(0, out)
+        engine._rollout_id = getattr(engine, "_rollout_id", 0) + 1
+        ctx.rollout_id = engine._rollout_id
         ctx.engine = engine
         ctx.K = mu.numel()
+        ctx.param_meta = [(t.device, t.dtype, t.shape) for t in (mu, sigma, weights)]
         return out
 
     @staticmethod
     def backward(ctx, grad_out):
         eng = ctx.engine
+        if getattr(eng, "_rollout_id", None) != ctx.rollout_id:
+            raise RuntimeError("the engine ran another rollout forward since this one: call backward "
+                               "before the next forward on the same engine")
         g = grad_out.detach().to(torch.float32).contiguous()
         torch.cuda.current_stream().synchronize()
         eng.rollout_backward([g])
@@ -42,7 +50,8 @@ class DiffusionLeniaRollout(torch.autograd.Function):
         gs = torch.empty_like(g)
         gm, gsg, gw = (torch.empty(ctx.K, dtype=torch.float32, device=dev) for _ in range(3))
         eng.creature_grads(0, gs, gm, gsg, gw, device=True)
-        return gs, gm, gsg, gw, None, None, None
+        gp = [t.to(device=d, dtype=dt).reshape(sh) for t, (d, dt, sh) in zip((gm, gsg, gw), ctx.param_meta)]
+        return (gs, *gp, None, None, None)
 
 
 def rollout(engine, state0, mu, sigma, weights, steps, ckpt_every=8):

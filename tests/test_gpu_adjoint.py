@@ -259,3 +259,26 @@ def test_rollout_grad_maximum_channels_and_kernels(cn):
     assert errs["loss"] <= 1e-5
     for k in ("state0", "mu", "sigma", "weights"):
         assert errs[k] <= 1e-4, (k, errs[k])
+
+
+def test_autograd_function_cpu_parameters_and_superseded_forward(cn):
+    """Growth parameters given as CPU float64 tensors get their gradients as CPU float64 tensors;
+    a backward whose forward was superseded by another forward on the same engine raises."""
+    from cella_nova_amd.autograd import rollout
+    cr, tg = _pair(cn, 64, 3, R=9, seed=43, patch=32)
+    eng = cn.make_engine(cr)
+    s0 = torch.as_tensor(cr["state"], device=DEV).requires_grad_(True)
+    mu, sg, w = (torch.as_tensor(cr[k].astype(np.float64)).requires_grad_(True) for k in ("mu", "sigma", "weights"))
+    tgt = torch.as_tensor(tg["state"], device=DEV)
+    loss = ((rollout(eng, s0, mu, sg, w, steps=3, ckpt_every=2) - tgt) ** 2).mean()
+    loss.backward()
+    _, g, _ = _engine_grads(cn, cr, tg["state"], 3, 2)
+    for name, t in (("mu", mu), ("sigma", sg), ("weights", w)):
+        assert t.grad.device.type == "cpu" and t.grad.dtype == torch.float64
+        assert _rel(t.grad.numpy(), g[name].astype(np.float64)) <= 1e-5
+    first = ((rollout(eng, s0, mu, sg, w, steps=3, ckpt_every=2) - tgt) ** 2).mean()
+    second = ((rollout(eng, s0, mu, sg, w, steps=4, ckpt_every=2) - tgt) ** 2).mean()
+    with pytest.raises(RuntimeError, match="another rollout forward"):
+        first.backward()
+    second.backward()
+    eng.close()
