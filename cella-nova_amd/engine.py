@@ -29,11 +29,22 @@ def _desc(n, C, K, src, dst, mu, sigma, w, temp, device, keep):
                       float(temp), int(device))
 
 
-def _ptr(x):
-    """Device address of a torch tensor, or host address of a numpy array."""
-    if hasattr(x, "data_ptr"):
-        return x.data_ptr()
-    return x.ctypes.data
+def _dev_f32(t, numel, what):
+    """Device address of `t`, which must be a contiguous float32 device tensor of `numel`
+    elements (the C ABI copies numel * 4 raw bytes)."""
+    if not hasattr(t, "data_ptr") or not t.is_cuda:
+        raise TypeError(f"{what} must be a device tensor")
+    if str(t.dtype) != "torch.float32" or not t.is_contiguous():
+        raise TypeError(f"{what} must be a contiguous float32 tensor, got {t.dtype}")
+    if t.numel() != numel:
+        raise ValueError(f"{what} has {t.numel()} elements, expected {numel}")
+    return t.data_ptr()
+
+
+def _host_f32(a, numel, what):
+    if not isinstance(a, np.ndarray) or a.dtype != np.float32 or not a.flags.c_contiguous or a.size != numel:
+        raise TypeError(f"{what} must be a contiguous float32 numpy array of {numel} elements")
+    return a.ctypes.data
 
 
 class _Engine:
@@ -105,10 +116,12 @@ class _Engine:
 
     def creature_state_from_device(self, i, tensor):
         """Copy a contiguous f32 device tensor (C, N, N) into the creature's current state."""
-        check(lib().cn_creature_state_from_device(self._h, int(i), tensor.data_ptr(), tensor.numel()))
+        n = self._elems(i)
+        check(lib().cn_creature_state_from_device(self._h, int(i), _dev_f32(tensor, n, "tensor"), n))
 
     def creature_state_to_device(self, i, tensor):
-        check(lib().cn_creature_state_to_device(self._h, int(i), tensor.data_ptr(), tensor.numel()))
+        n = self._elems(i)
+        check(lib().cn_creature_state_to_device(self._h, int(i), _dev_f32(tensor, n, "tensor"), n))
 
     def creature_set_target(self, i, target):
         a = _f32(target).reshape(-1)
@@ -120,7 +133,8 @@ class _Engine:
             arr = np.empty(self._elems(i), dtype=np.float32)
             check(lib().cn_creature_rollout_final(self._h, int(i), arr.ctypes.data, arr.size, 0))
             return arr
-        check(lib().cn_creature_rollout_final(self._h, int(i), out.data_ptr(), out.numel(), 1))
+        n = self._elems(i)
+        check(lib().cn_creature_rollout_final(self._h, int(i), _dev_f32(out, n, "out"), n, 1))
         return out
 
     def creature_loss(self, i):
@@ -132,13 +146,14 @@ class _Engine:
         """Gradients of the last backward.  Host: returns numpy arrays (dL/dstate0 only if
         with_state).  device=True: fills the given device tensors."""
         C, n, K = self._shapes[i]
+        ptr = _dev_f32 if device else _host_f32
         if not device:
             state = np.empty(C * n * n, np.float32) if with_state else None
             mu, sigma, w = (np.empty(K, np.float32) for _ in range(3))
         check(lib().cn_creature_get_grads(
-            self._h, int(i), _ptr(state) if state is not None else None, C * n * n,
-            _ptr(mu) if mu is not None else None, _ptr(sigma) if sigma is not None else None,
-            _ptr(w) if w is not None else None, K, 1 if device else 0))
+            self._h, int(i), ptr(state, C * n * n, "state") if state is not None else None, C * n * n,
+            ptr(mu, K, "mu") if mu is not None else None, ptr(sigma, K, "sigma") if sigma is not None else None,
+            ptr(w, K, "w") if w is not None else None, K, 1 if device else 0))
         return {"state0": state, "mu": mu, "sigma": sigma, "weights": w}
 
     def creature_generate(self, i, radius, patch=128, seed=0, creature=0, src=None, dst=None):
@@ -189,7 +204,10 @@ class _Engine:
 
     def rollout_backward(self, grad_finals):
         """dL/dx_T per creature as contiguous f32 device tensors."""
-        arr = (ctypes.c_void_p * len(grad_finals))(*[g.data_ptr() for g in grad_finals])
+        if len(grad_finals) != len(self._shapes):
+            raise ValueError(f"expected {len(self._shapes)} gradients, got {len(grad_finals)}")
+        arr = (ctypes.c_void_p * len(grad_finals))(
+            *[_dev_f32(g, self._elems(i), f"grad_finals[{i}]") for i, g in enumerate(grad_finals)])
         check(lib().cn_rollout_backward(self._h, arr, len(grad_finals)))
 
     def autotune(self, reps=5):

@@ -143,3 +143,22 @@ def test_creature_rejects_bad_params(cn):
         cn.generate_creature(100, 1, 1, [0], [0], 5)
     with pytest.raises(cn.CellaNovaError):
         cn.generate_creature(64, 2, 1, [2], [0], 5)
+
+
+def test_python_mirror_checks_tensor_arguments(cn):
+    """Device-pointer entry points copy numel * 4 raw bytes: the Python mirror rejects anything
+    but contiguous float32 device tensors of the right size before calling them (no GPU needed:
+    the checks come first)."""
+    torch = pytest.importorskip("torch")
+    from cella_nova_amd.engine import _dev_f32, _host_f32
+    with pytest.raises(TypeError):
+        _dev_f32(torch.zeros(4), 4, "t")                      # host tensor
+    with pytest.raises(TypeError):
+        _dev_f32(np.zeros(4, np.float32), 4, "t")             # numpy array
+    with pytest.raises(TypeError):
+        _host_f32(np.zeros(4, np.float64), 4, "a")            # wrong dtype
+    with pytest.raises(TypeError):
+        _host_f32(np.zeros(8, np.float32)[::2], 4, "a")       # not contiguous
+    with pytest.raises(TypeError):
+        _host_f32(np.zeros(5, np.float32), 4, "a")            # wrong size
+    assert _host_f32(np.zeros(4, np.float32), 4, "a") != 0
