@@ -10,6 +10,9 @@
 
 namespace cn {
 
+// Error text of the calling thread (errors.cpp): a fixed buffer of kErrorTextSize bytes, so
+// recording an error never allocates (it also works while reporting a failed allocation).
+constexpr size_t kErrorTextSize = 1024;
 cn_status set_error(cn_status st, const char* fmt, ...);
 void clear_error();
 const char* last_error_text();

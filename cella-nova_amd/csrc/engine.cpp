@@ -24,37 +24,8 @@ namespace cn {
 // ---------------------------------------------------------------------------------------
 // errors
 // ---------------------------------------------------------------------------------------
-// A fixed per-thread buffer: recording an error never allocates, so it also works while
-// reporting a failed host allocation.
-static thread_local char g_err[1024];
 static int g_fail_next = 0;
 static int g_fail_at = -1;  // checked calls to let through before one injected failure
-
-cn_status set_error(cn_status st, const char* fmt, ...) {
-  char buf[sizeof g_err];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
-  va_end(ap);
-  std::memcpy(g_err, buf, sizeof buf);
-  return st;
-}
-void clear_error() { g_err[0] = 0; }
-const char* last_error_text() { return g_err; }
-
-cn_status status_of_exception() noexcept {
-  try {
-    throw;
-  } catch (const std::bad_alloc&) {
-    return set_error(CN_ERR_OUT_OF_MEMORY, "host allocation failed");
-  } catch (const std::length_error&) {
-    return set_error(CN_ERR_OUT_OF_MEMORY, "host allocation too large");
-  } catch (const std::exception& ex) {
-    return set_error(CN_ERR_STATE, "internal error: %s", ex.what());
-  } catch (...) {
-    return set_error(CN_ERR_STATE, "internal error");
-  }
-}
 
 bool inject_failure() {
   if (g_fail_next > 0) {
@@ -634,7 +605,7 @@ cn_status create_engine(const cn_engine_desc* descs, uint32_t count, int device,
         validate_creature(d.n, d.num_channels, d.num_kernels, d.src, d.dst, d.mu, d.sigma, d.weights, d.temp);
     if (st != CN_OK) {
       if (count > 1) {
-        char msg[sizeof g_err];
+        char msg[kErrorTextSize];
         std::memcpy(msg, last_error_text(), sizeof msg);
         set_error(st, "creature %u: %s", i, msg);
       }
@@ -734,8 +705,6 @@ float* cur_state(cn_engine* e, uint32_t i) {
 }  // namespace
 
 extern "C" {
-
-const char* cn_last_error(void) { return g_err; }
 
 void cn_debug_fail_next_hip(int count) { g_fail_next = count > 0 ? count : 0; }
 

@@ -87,6 +87,7 @@ struct Reader {
     errno = 0;
     v = std::strtod(tok.c_str(), &e);
     if (e != tok.c_str() + tok.size()) return fail("malformed number");
+    if (!std::isfinite(v)) return fail("number out of range");  // as serde_json: no f64 overflow
     return true;
   }
   bool skip_value(int depth = 0) {

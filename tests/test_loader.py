@@ -60,6 +60,13 @@ def test_load_config_malformed_and_missing(cn, tmp_path):
     with pytest.raises(cn.CellaNovaError) as ei:
         cn.load_config(p)
     assert ei.value.status == 5
+    # a number beyond the f64 range is a parse error (serde_json refuses it), not an infinity
+    p.write_text(json.dumps(_cfg()).replace("105.0", "1e400"))
+    with pytest.raises(cn.CellaNovaError) as ei:
+        cn.load_config(p)
+    assert ei.value.status == 5 and "out of range" in str(ei.value)
+    p.write_text(json.dumps(_cfg()).replace("105.0", "1e-400"))  # underflow reads as 0, as serde_json
+    assert cn.load_config(p).seed_channels[1][5] == 0.0
     with pytest.raises(cn.CellaNovaError) as ei:
         cn.load_config(tmp_path / "nope.json")
     assert ei.value.status == 4
