@@ -244,10 +244,16 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
       const float w2 = 2.0f * wk;
       wsum += wk;
       float2 v[E];
+      // Row loads at raised priority: without a prefetch they gate this wave's progress, and
+      // issued ahead of the other waves' transform work they overlap it (2 % on the pass).
+      __builtin_amdgcn_s_setprio(1);
       if constexpr (!kPrefetch) c2r_load_own<N, E>(uhat + ((size_t)k * N + y) * L, t, xp);
+      __builtin_amdgcn_s_setprio(0);
       c2r_partner<N, E, ROWS>(xp, lds, t, lane, xq);
       c2r_form<N, E, true>(xp, xq, t, twl + TI::size, v);
+      __builtin_amdgcn_s_setprio(1);
       if (kPrefetch && kk + 1 < ke) c2r_load_own<N, E>(uhat + ((size_t)d.dst_k[kk + 1] * N + y) * L, t, xp);
+      __builtin_amdgcn_s_setprio(0);
       fft_run<L, E, ROWS, false, true, kTwN, 0, true>(v, lds, t, lane, twl);
 #pragma unroll
       for (int i = 0; i < E; ++i) {
