@@ -393,7 +393,7 @@ def main():
     roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(tag, dom)}
     step_alg = sum(bpc.values()) * n * n
-    survey_model = (36 * C + 32 * K) * n * n  # C2C model: 396 B/cell-step at C3K9
+    c2c_model = (36 * C + 32 * K) * n * n  # C2C model: 396 B/cell-step at C3K9
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
@@ -405,8 +405,8 @@ def main():
         "kernels": per_kernel,
         "step_roofline": {"alg_bytes_per_step": step_alg, "GBps": round(step_alg / (ms_per_step * 1e-3) / 1e9, 1),
                           "frac": round(step_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                          "c2c_model_bytes_per_step": survey_model,
-                          "c2c_model_frac": round(survey_model / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                          "c2c_model_bytes_per_step": c2c_model,
+                          "c2c_model_frac": round(c2c_model / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                           "c2c_model_note": "bytes of the full-complex model (36C+32K B/cell) per step time; this "
                                             "design moves fewer bytes, so above 1 means the model is beaten"},
         "mass_drift_rel_fp64": drift,
