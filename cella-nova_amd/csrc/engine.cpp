@@ -599,6 +599,9 @@ cn_status create_engine(const cn_engine_desc* descs, uint32_t count, int device,
   if (!descs || !out) return set_error(CN_ERR_INVALID_ARG, "null descs or out");
   *out = nullptr;
   if (count == 0) return set_error(CN_ERR_INVALID_ARG, "creature count must be > 0");
+  // The creatures of a size class are the z dimension of its launch grids.
+  if (count > kMaxCreatures)
+    return set_error(CN_ERR_UNSUPPORTED, "creature count %u exceeds %u", count, kMaxCreatures);
   for (uint32_t i = 0; i < count; ++i) {
     const cn_engine_desc& d = descs[i];
     const cn_status st =

@@ -35,6 +35,7 @@ namespace cn {
 
 constexpr int kMaxChannels = 16;   // reference growth shader limit: array<f32,16>
 constexpr int kMaxKernels = 256;
+constexpr unsigned kMaxCreatures = 65535;  // per engine: the grid z dimension of a launch
 constexpr int kTwN = 4096;         // twiddle table length; supports N <= 4096
 
 // Twiddle table layout: kTwN entries tw[i] = exp(-2 pi i i / kTwN), then one block per Stockham

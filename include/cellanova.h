@@ -129,7 +129,8 @@ const char* cn_last_error(void);
 /* An engine holding `count` creatures (descs[i].device is ignored; `device` is used).  Every
  * step / rollout call advances all creatures together; creatures of equal N share one launch
  * per kernel (one descriptor table per size class) and all classes are captured in one HIP
- * graph.  The single-creature calls above act on creature 0. */
+ * graph.  The single-creature calls above act on creature 0.  At most 65535 creatures
+ * (CN_ERR_UNSUPPORTED beyond). */
 cn_status cn_batch_create(const cn_engine_desc* descs, uint32_t count, int device, cn_engine** out);
 /* Per-creature forms of set_kernel / upload_channel / download_all_channels / mass. */
 cn_status cn_creature_set_kernel(cn_engine* e, uint32_t creature, uint32_t k, const float* interleaved,

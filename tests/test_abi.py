@@ -71,6 +71,17 @@ def test_null_arguments_rejected(cn):
     assert lib.cn_load_kernels(None, 1, 16, None, 0) == 1
 
 
+def test_batch_size_limit_checked_before_hip(cn):
+    """More creatures than a launch grid's z dimension holds is refused up front (the
+    descriptors beyond the first are not read)."""
+    lib = cn.lib()
+    d, keep = _desc(cn)
+    h = ctypes.c_void_p()
+    assert lib.cn_batch_create(ctypes.byref(d), 65536, 0, ctypes.byref(h)) == 2
+    assert b"65535" in lib.cn_last_error()
+    assert h.value is None
+
+
 def test_injected_hip_failure_is_reported(cn):
     lib = cn.lib()
     d, keep = _desc(cn)
