@@ -398,10 +398,15 @@ static hipError_t launch_adj_rows(const StepLaunch& sl, int gout, hipStream_t st
     k_growth_adj_row<N, ER><<<dim3(CF::BLOCKS, sl.max_k, sl.count), CF::THREADS, 0, st>>>(sl.descs, sl.tw, sl.islot);
   } else {
     k_row_inv_add<N, ER><<<dim3(CF::BLOCKS, sl.max_c, sl.count), CF::THREADS, 0, st>>>(sl.descs, gout, sl.tw);
-    k_param_reduce<<<dim3((3 * sl.max_k + kReduceWaves - 1) / kReduceWaves, 1, sl.count), 64 * kReduceWaves, 0, st>>>(
-        sl.descs, CF::BLOCKS);
   }
   return hipSuccess;
+}
+
+// Parameter-gradient reduction of the growth adjoint's per-block partials (row geometry ER).
+template <int N, int ER>
+static void launch_param_reduce(const StepLaunch& sl, hipStream_t st) {
+  k_param_reduce<<<dim3((3 * sl.max_k + kReduceWaves - 1) / kReduceWaves, 1, sl.count), 64 * kReduceWaves, 0, st>>>(
+      sl.descs, RowCfg<N, ER>::BLOCKS);
 }
 
 // One adjoint step: (sl.adj_recompute) recompute the forward intermediates of state slot sl.sin
@@ -440,6 +445,13 @@ static hipError_t launch_adjoint_n(const StepLaunch& sl, int gin, int gout, hipS
   if (tc > 4) tc = 4;
   if ((e = launch_col_adj_tc<N, 8>(sl, tc, st)) != hipSuccess) return e;
   if ((e = rows(false)) != hipSuccess) return e;
+  // Parameter-gradient reduction over the partials of the growth adjoint, whose row geometry
+  // (the plan's forward one) sets their block count; the inverse row geometry may differ.
+  switch (sl.fwd_e ? sl.fwd_e : DefaultGeom<N>::FWD_E) {
+    case 8: launch_param_reduce<N, 8>(sl, st); break;
+    case 16: launch_param_reduce<N, 16>(sl, st); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

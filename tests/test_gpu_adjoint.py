@@ -282,3 +282,24 @@ def test_autograd_function_cpu_parameters_and_superseded_forward(cn):
         first.backward()
     second.backward()
     eng.close()
+
+
+@pytest.mark.parametrize("n", [256, 1024])
+def test_rollout_grad_mixed_row_geometries(cn, n):
+    """Forward and inverse row transforms with different values per thread (different row-block
+    counts): the parameter-gradient reduction reads exactly the partials the growth adjoint of
+    this plan wrote, also after a plan with more row blocks left its partials in the buffer."""
+    cr, tg = _pair(cn, n, 3, R=max(4, n // 20), seed=47, patch=min(128, n // 2))
+    loss_t, gs_t, gm_t, gsg_t, gw_t = _torch_grads(cr, tg["state"], 3)
+    eng = cn.make_engine(cr)
+    eng.creature_set_target(0, tg["state"])
+    for fe, ie in ((8, 8), (16, 8), (8, 16), (16, 16), (8, 16)):
+        eng.set_plan(n, fwd_e=fe, inv_e=ie)
+        eng.rollout_grad(3, 2)
+        g = eng.creature_grads(0)
+        errs = {"state0": _rel(g["state0"].reshape(gs_t.shape), gs_t), "mu": _rel(g["mu"], gm_t),
+                "sigma": _rel(g["sigma"], gsg_t), "weights": _rel(g["weights"], gw_t)}
+        print(f"n={n} fwd_e={fe} inv_e={ie}: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+        for k, v in errs.items():
+            assert v <= 1e-4, (fe, ie, k, v)
+    eng.close()
