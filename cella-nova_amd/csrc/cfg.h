@@ -22,14 +22,17 @@ struct ColCfg {
 };
 
 // Built-in launch geometry (used when no tuned plan is given).
+constexpr int default_col_tc(int N) {
+  const int tc = N >= 2048 ? 4 : (N >= 1024 ? 8 : 16);
+  return tc < N / 2 ? tc : N / 2;
+}
 template <int N>
 struct DefaultGeom {
   static constexpr int FWD_E = 8;
   static constexpr int INV_E = 8;
   static constexpr int COL_E = 16;
   static constexpr int DIFF_TH = 16;
-  static constexpr int TC_DEF = N >= 2048 ? 4 : (N >= 1024 ? 8 : 16);
-  static constexpr int TC = TC_DEF < N / 2 ? TC_DEF : N / 2;
+  static constexpr int TC = default_col_tc(N);
 };
 
 // Column geometries a launcher may pick for size N: at most 1024 threads per block and the

@@ -1517,8 +1517,8 @@ cn_status cn_engine_set_plan(cn_engine* e, const cn_plan* plan) {
     return set_error(CN_ERR_INVALID_ARG, "diffusion tile height must be 0, 8, 16 or 32");
   if (plan->col_split > 1) return set_error(CN_ERR_INVALID_ARG, "col_split must be 0 or 1");
   if (plan->col_tc != 0 || plan->col_e != 0) {
-    const int ce = plan->col_e ? (int)plan->col_e : 16;
-    const int tc = plan->col_tc ? (int)plan->col_tc : 4;
+    const int ce = plan->col_e ? (int)plan->col_e : 16;  // the launcher's defaults (DefaultGeom)
+    const int tc = plan->col_tc ? (int)plan->col_tc : default_col_tc(k->N);
     if (!col_geom_ok(k->N, ce, tc))
       return set_error(CN_ERR_INVALID_ARG, "column geometry (E=%d, TC=%d) does not fit N=%d", ce, tc, k->N);
   }
