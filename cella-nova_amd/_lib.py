@@ -75,6 +75,9 @@ SIGNATURES = {
     "cn_iterate": (c_int, [c_void_p]),
     "cn_run_steps": (c_int, [c_void_p, c_uint64]),
     "cn_sync": (c_int, [c_void_p]),
+    "cn_iterate_on_stream": (c_int, [c_void_p, c_void_p]),
+    "cn_engine_stream": (c_int, [c_void_p, POINTER(c_void_p)]),
+    "cn_creature_state_ptr": (c_int, [c_void_p, c_uint32, POINTER(c_void_p), POINTER(c_size_t)]),
     "cn_timed_iterate": (c_int, [c_void_p, POINTER(c_double)]),
     "cn_profile_kernels": (c_int, [c_void_p, c_uint64, c_void_p, c_size_t]),
     "cn_download_channels": (c_int, [c_void_p, c_void_p, c_size_t]),

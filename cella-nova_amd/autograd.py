@@ -22,6 +22,14 @@ class DiffusionLeniaRollout(torch.autograd.Function):
             raise TypeError("engine must be a GpuMaceLenia")
         if state0.dtype != torch.float32 or not state0.is_cuda:
             raise TypeError("state0 must be a float32 device tensor")
+        C, n, K = engine._shapes[0]
+        if state0.numel() != C * n * n:
+            raise ValueError(f"state0 has {state0.numel()} elements, expected {C * n * n}")
+        for name, t in (("mu", mu), ("sigma", sigma), ("weights", weights)):
+            if t.numel() != K:
+                raise ValueError(f"{name} has {t.numel()} elements, expected {K}")
+        # From here on the engine is modified: an earlier forward's backward must not run on it.
+        engine._rollout_id = getattr(engine, "_rollout_id", 0) + 1
         engine.set_growth(mu.detach().cpu().numpy(), sigma.detach().cpu().numpy(),
                           weights.detach().cpu().numpy())
         s0 = state0.detach().contiguous()
@@ -30,7 +38,6 @@ class DiffusionLeniaRollout(torch.autograd.Function):
         engine.rollout_forward(steps, ckpt_every)
         out = torch.empty_like(s0)
         engine.creature_rollout_final(0, out)
-        engine._rollout_id = getattr(engine, "_rollout_id", 0) + 1
         ctx.rollout_id = engine._rollout_id
         ctx.engine = engine
         ctx.K = mu.numel()

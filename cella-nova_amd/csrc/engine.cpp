@@ -13,11 +13,13 @@
 #include <cstring>
 #include <new>
 #include <stdexcept>
+#include <type_traits>
 #include <vector>
 #include "common.h"
 #include "engine.h"
 #include "cfg.h"
 #include "launch.h"
+#include "mass.h"
 
 namespace cn {
 
@@ -202,12 +204,24 @@ struct SizeClass {
   bool col_split = false;
 };
 
+// Captured rollout graphs of one (T, checkpoint stride) plan.
 struct Rollout {
   uint32_t T = 0, ckpt = 0;
   hipGraphExec_t graph = nullptr;   // forward + loss + backward (targets)
   hipGraphExec_t fwd = nullptr;     // forward only
   int final_slot = 0;
-  int grad_slot = 0;                // gradient slot holding dL/dstate0 after the backward
+  int grad_slot = 0;                // gradient slot holding dL/dstate0 after the graph's backward
+};
+
+// What the last rollout calls left on the device.  Kept apart from the graphs: dropping the
+// graphs (a new launch plan) does not move any buffer, so these stay valid.
+struct RolloutResult {
+  uint32_t T = 0, ckpt = 0;         // plan of the last successful forward
+  int final_slot = 0;               // state slot holding x_T of that forward
+  bool fwd_valid = false;           // its checkpoints and x_T are intact
+  int grad_slot = 0;                // gradient slot holding dL/dstate0 of the last backward
+  bool grads_valid = false;         // a backward completed (gradients readable)
+  bool loss_valid = false;          // a cn_rollout_grad completed (loss partials readable)
 };
 
 }  // namespace
@@ -224,25 +238,39 @@ struct cn_engine {
   uint64_t bytes = 0;
   float2* tw = nullptr;
   double2* mass_partial = nullptr;
-  int mass_nblk = 64;
+  int mass_nblk = kMassBlocks;
   std::vector<void*> allocs;
   hipGraphExec_t step_graph[2] = {nullptr, nullptr};
   hipGraphExec_t render_graph[2] = {nullptr, nullptr};  // step + render epilogue
   hipGraphExec_t multi_graph[2] = {nullptr, nullptr};   // kMultiSteps steps (run_steps)
   RingParams* gen_rings = nullptr;  // device generator parameters, kMaxKernels entries
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  Rollout ro;               // last rollout plan
-  bool ro_fwd_done = false; // a rollout forward has run since the last state change
+  hipEvent_t join[2] = {nullptr, nullptr};  // cross-stream ordering (cn_iterate_on_stream)
+  Rollout ro;               // last captured rollout plan
+  RolloutResult res;        // results of the last rollout forward / backward
 };
 
 namespace {
 
+// Zeroed device allocation owned by `owner`.  On failure nothing is kept or counted and *p is
+// left null.
 cn_status dalloc(cn_engine* e, std::vector<void*>& owner, void** p, size_t bytes) {
+  *p = nullptr;
   if (inject_failure()) return set_error(CN_ERR_HIP, "hipMalloc: injected failure");
-  CN_HIP(hipMalloc(p, bytes));
-  owner.push_back(*p);
+  void* q = nullptr;
+  CN_HIP(hipMalloc(&q, bytes));
+  cn_status st;
+  if (inject_failure()) {
+    (void)hipFree(q);
+    return set_error(CN_ERR_HIP, "hipMemsetAsync: injected failure");
+  }
+  if (hip_failed(hipMemsetAsync(q, 0, bytes, e->stream), "hipMemsetAsync", &st)) {
+    (void)hipFree(q);
+    return st;
+  }
+  owner.push_back(q);
   e->bytes += bytes;
-  CN_HIP(hipMemsetAsync(*p, 0, bytes, e->stream));
+  *p = q;
   return CN_OK;
 }
 
@@ -255,7 +283,7 @@ void drop_graphs(cn_engine* e) {
       }
   if (e->ro.graph) (void)hipGraphExecDestroy(e->ro.graph);
   if (e->ro.fwd) (void)hipGraphExecDestroy(e->ro.fwd);
-  e->ro = Rollout{};
+  e->ro = Rollout{};  // e->res is untouched: no buffer moves with the graphs
 }
 
 void release(cn_engine* e) {
@@ -264,6 +292,8 @@ void release(cn_engine* e) {
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   drop_graphs(e);
   for (auto& v : e->ev)
+    if (v) (void)hipEventDestroy(v);
+  for (auto& v : e->join)
     if (v) (void)hipEventDestroy(v);
   for (auto& c : e->cr)
     for (void* p : c.allocs) (void)hipFree(p);
@@ -356,18 +386,29 @@ cn_status capture(cn_engine* e, hipGraphExec_t* out, F&& fn) {
 
 constexpr int kMultiSteps = 8;  // steps per run_steps graph (even)
 
-cn_status enqueue_step(cn_engine* e, bool render = false) {
+// One step as a graph launch on `on` (the engine's stream unless a caller's stream is given:
+// then the caller's stream first waits for the engine's queued work and the engine's stream
+// afterwards waits for the step, so both stay ordered without a host wait).
+cn_status enqueue_step(cn_engine* e, bool render = false, hipStream_t on = nullptr, bool foreign = false) {
   const int cur = e->cur;
   hipGraphExec_t* g = render ? &e->render_graph[cur] : &e->step_graph[cur];
   if (!*g) {
     const cn_status st = capture(e, g, [&] { return launch_all_steps(e, cur, cur ^ 1, render); });
     if (st != CN_OK) return st;
   }
-  CN_HIP(hipGraphLaunch(*g, e->stream));
+  if (foreign) {
+    CN_HIP(hipEventRecord(e->join[0], e->stream));
+    CN_HIP(hipStreamWaitEvent(on, e->join[0], 0));
+    CN_HIP(hipGraphLaunch(*g, on));
+    CN_HIP(hipEventRecord(e->join[1], on));
+    CN_HIP(hipStreamWaitEvent(e->stream, e->join[1], 0));
+  } else {
+    CN_HIP(hipGraphLaunch(*g, e->stream));
+  }
   e->cur ^= 1;
   e->steps++;
   e->have_prev = true;
-  e->ro_fwd_done = false;
+  e->res.fwd_valid = false;
   return CN_OK;
 }
 
@@ -383,6 +424,8 @@ cn_status ensure_slots(cn_engine* e, int want) {
   CN_HIP(hipStreamSynchronize(e->stream));
   for (size_t i = 0; i < e->cr.size(); ++i) {
     Creature& c = e->cr[i];
+    // A creature already grown by an earlier call that failed part-way keeps its buffer.
+    if (c.dev.nslots >= want) continue;
     void* p = nullptr;
     const cn_status st = dalloc(e, c.allocs, &p, (size_t)want * c.slot_elems() * 4);
     if (st != CN_OK) return st;
@@ -403,7 +446,7 @@ cn_status ensure_slots(cn_engine* e, int want) {
     auto it = std::find(c.allocs.begin(), c.allocs.end(), (void*)old);
     if (it != c.allocs.end()) {
       CN_HIP(hipFree(*it));
-      e->bytes -= (size_t)e->nslots * c.slot_elems() * 4;
+      e->bytes -= (size_t)old_n * c.slot_elems() * 4;  // the size of the buffer just freed
       c.allocs.erase(it);
     }
   }
@@ -423,22 +466,22 @@ cn_status ensure_adjoint(cn_engine* e) {
       if ((st = dalloc(e, c.allocs, &p, K * P * 4)) != CN_OK) return st;
       d.u = (float*)p;
     }
-    if ((st = dalloc(e, c.allocs, &p, 2 * C * P * 4)) != CN_OK) return st;
-    d.grad = (float*)p;
-    if ((st = dalloc(e, c.allocs, &p, C * P * 4)) != CN_OK) return st;
-    d.gaff = (float*)p;
-    if ((st = dalloc(e, c.allocs, &p, C * P * 4)) != CN_OK) return st;
-    d.dsd = (float*)p;
-    if ((st = dalloc(e, c.allocs, &p, K * c.N * H * 8)) != CN_OK) return st;
-    d.specg = p;
-    if ((st = dalloc(e, c.allocs, &p, (size_t)row_blocks((int)c.N) * K * 3 * 4)) != CN_OK) return st;
-    d.partial = (float*)p;
-    if ((st = dalloc(e, c.allocs, &p, 6 * K * 4)) != CN_OK) return st;
-    d.gparam = (float*)p;
-    if ((st = dalloc(e, c.allocs, &p, C * P * 4)) != CN_OK) return st;
-    d.target = (float*)p;
-    if ((st = dalloc(e, c.allocs, &p, (size_t)kLossPartials * 4)) != CN_OK) return st;
-    d.lossp = (float*)p;
+    // Each buffer is allocated once: a retry after a failure part-way keeps the ones it has.
+    auto need = [&](auto*& field, size_t bytes) -> cn_status {
+      if (field) return CN_OK;
+      void* q = nullptr;
+      const cn_status s2 = dalloc(e, c.allocs, &q, bytes);
+      if (s2 == CN_OK) field = (std::remove_reference_t<decltype(field)>)q;
+      return s2;
+    };
+    if ((st = need(d.grad, 2 * C * P * 4)) != CN_OK) return st;
+    if ((st = need(d.gaff, C * P * 4)) != CN_OK) return st;
+    if ((st = need(d.dsd, C * P * 4)) != CN_OK) return st;
+    if ((st = need(d.specg, K * c.N * H * 8)) != CN_OK) return st;
+    if ((st = need(d.partial, (size_t)row_blocks((int)c.N) * K * 3 * 4)) != CN_OK) return st;
+    if ((st = need(d.gparam, 6 * K * 4)) != CN_OK) return st;
+    if ((st = need(d.target, C * P * 4)) != CN_OK) return st;
+    if ((st = need(d.lossp, (size_t)kLossPartials * 4)) != CN_OK) return st;
     c.adj = true;
     if ((st = upload_desc(e, (int)i)) != CN_OK) return st;
   }
@@ -497,6 +540,7 @@ struct SlotPlan {
   int segslot(int i) const { return seg0 + i - 1; }
 };
 SlotPlan plan_slots(uint32_t T, uint32_t ck) {
+  if (ck == 0) ck = 1;  // callers validate T, ck > 0; never divide by zero
   SlotPlan p;
   p.nck = (int)((T + ck - 1) / ck);
   p.x0 = 2 + p.nck - 1;
@@ -557,6 +601,9 @@ hipError_t record_backward(cn_engine* e, const SlotPlan& p, uint32_t T, uint32_t
 cn_status prepare_rollout(cn_engine* e, uint32_t T, uint32_t ck) {
   if (T == 0 || ck == 0) return set_error(CN_ERR_INVALID_ARG, "steps and checkpoint stride must be > 0");
   if (ck > T) ck = T;
+  // The forward about to run overwrites the previous one's checkpoints (and growing the slot
+  // arrays below keeps only slots 0 and 1): its results are gone whatever happens next.
+  e->res.fwd_valid = false;
   cn_status st;
   if ((st = ensure_adjoint(e)) != CN_OK) return st;
   const SlotPlan p = plan_slots(T, ck);
@@ -595,6 +642,15 @@ cn_status normalise_current(cn_engine* e) {
   return CN_OK;
 }
 
+// Records the plan of the rollout forward just enqueued (its checkpoints and x_T are valid
+// until the state, the parameters or the kernels change, or another rollout runs).
+void set_forward_result(cn_engine* e) {
+  e->res.T = e->ro.T;
+  e->res.ckpt = e->ro.ckpt;
+  e->res.final_slot = e->ro.final_slot;
+  e->res.fwd_valid = true;
+}
+
 cn_status create_engine(const cn_engine_desc* descs, uint32_t count, int device, cn_engine** out) {
   if (!descs || !out) return set_error(CN_ERR_INVALID_ARG, "null descs or out");
   *out = nullptr;
@@ -629,6 +685,8 @@ cn_status create_engine(const cn_engine_desc* descs, uint32_t count, int device,
   if (hip_failed(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate", &st)) return fail(st);
   for (auto& v : e->ev)
     if (hip_failed(hipEventCreate(&v), "hipEventCreate", &st)) return fail(st);
+  for (auto& v : e->join)
+    if (hip_failed(hipEventCreateWithFlags(&v, hipEventDisableTiming), "hipEventCreate", &st)) return fail(st);
 
   // size classes in order of first appearance
   e->cr.resize(count);
@@ -751,6 +809,7 @@ cn_status cn_creature_set_kernel(cn_engine* e, uint32_t i, uint32_t k, const flo
   if (!finite_all(interleaved, 2 * n_complex)) return set_error(CN_ERR_INVALID_ARG, "kernel spectrum not finite");
   std::vector<float> half(N * H * 2), nyq(N * 2);
   project_kernel(interleaved, c.N, half.data(), nyq.data());
+  e->res.fwd_valid = false;  // the next backward would recompute with the new spectrum
   std::vector<float> real;
   const bool sym = real_even_spectrum(half.data(), nyq.data(), c.N, real);
   CN_HIP(hipSetDevice(e->device));
@@ -791,7 +850,7 @@ cn_status cn_creature_upload_channel_f32(cn_engine* e, uint32_t i, uint32_t chan
   CN_HIP(hipMemcpyAsync(cur_state(e, i) + channel * P, data, P * 4, hipMemcpyHostToDevice, e->stream));
   CN_HIP(hipStreamSynchronize(e->stream));
   e->have_prev = false;
-  e->ro_fwd_done = false;
+  e->res.fwd_valid = false;
   return CN_OK;
   CN_CATCH
 }
@@ -829,6 +888,8 @@ cn_status cn_creature_set_growth(cn_engine* e, uint32_t i, const float* mu, cons
   Creature& c = e->cr[i];
   if (K != c.K) return set_error(CN_ERR_INVALID_ARG, "expected %u growth parameters, got %zu", c.K, K);
   if ((st = validate_growth(c.K, mu, sigma, w)) != CN_OK) return st;
+  // A backward must recompute its segments with the parameters of its own forward.
+  e->res.fwd_valid = false;
   fill_growth(c.dev, c.K, mu, sigma, w);
   CN_HIP(hipSetDevice(e->device));
   return upload_desc(e, (int)i);
@@ -874,7 +935,7 @@ cn_status cn_creature_generate(cn_engine* e, uint32_t i, const cn_creature_param
   CN_HIP(launch_seed(d, e->cur, c.slot_elems(), sp.side, sp.origin, p->seed, p->creature, e->stream));
   CN_HIP(hipStreamSynchronize(e->stream));
   e->have_prev = false;
-  e->ro_fwd_done = false;
+  e->res.fwd_valid = false;
   return CN_OK;
   CN_CATCH
 }
@@ -954,7 +1015,7 @@ cn_status cn_run_steps(cn_engine* e, uint64_t n) {
     CN_HIP(hipGraphLaunch(e->multi_graph[cur], e->stream));
     e->steps += kMultiSteps;
     e->have_prev = true;
-    e->ro_fwd_done = false;
+    e->res.fwd_valid = false;
     n -= kMultiSteps;
   }
   for (uint64_t j = 0; j < n; ++j) {
@@ -971,6 +1032,37 @@ cn_status cn_sync(cn_engine* e) {
   if (!e) return set_error(CN_ERR_INVALID_ARG, "null engine");
   CN_HIP(hipSetDevice(e->device));
   CN_HIP(hipStreamSynchronize(e->stream));
+  return CN_OK;
+  CN_CATCH
+}
+
+cn_status cn_iterate_on_stream(cn_engine* e, void* stream) {
+  CN_TRY
+  clear_error();
+  if (!e) return set_error(CN_ERR_INVALID_ARG, "null engine");
+  CN_HIP(hipSetDevice(e->device));
+  const hipStream_t s = (hipStream_t)stream;
+  return enqueue_step(e, false, s, s != e->stream);
+  CN_CATCH
+}
+
+cn_status cn_engine_stream(const cn_engine* e, void** stream) {
+  CN_TRY
+  clear_error();
+  if (!e || !stream) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  *stream = (void*)e->stream;
+  return CN_OK;
+  CN_CATCH
+}
+
+cn_status cn_creature_state_ptr(cn_engine* e, uint32_t i, float** d_state, size_t* n) {
+  CN_TRY
+  clear_error();
+  cn_status st = check_creature(e, i);
+  if (st != CN_OK) return st;
+  if (!d_state) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  *d_state = cur_state(e, i);
+  if (n) *n = e->cr[i].slot_elems();
   return CN_OK;
   CN_CATCH
 }
@@ -1049,7 +1141,7 @@ static cn_status timed_step(cn_engine* e, float kernel_ms[4]) {
   e->cur ^= 1;
   e->steps++;
   e->have_prev = true;
-  e->ro_fwd_done = false;
+  e->res.fwd_valid = false;
   return CN_OK;
 }
 
@@ -1164,18 +1256,8 @@ cn_status cn_creature_mass_fp64(cn_engine* e, uint32_t i, double* out, size_t n_
   CN_HIP(hipMemcpyAsync(part.data(), e->mass_partial + (size_t)c.idx * kMaxChannels * e->mass_nblk,
                         part.size() * 8, hipMemcpyDeviceToHost, e->stream));
   CN_HIP(hipStreamSynchronize(e->stream));
-  for (uint32_t ch = 0; ch < c.C; ++ch) {
-    double s = 0.0, comp = 0.0;
-    for (int b = 0; b < e->mass_nblk; ++b) {
-      const double* pr = &part[((size_t)ch * e->mass_nblk + b) * 2];
-      for (int j = 0; j < 2; ++j) {
-        const double x = pr[j], t = s + x;
-        comp += (std::fabs(s) >= std::fabs(x)) ? (s - t) + x : (x - t) + s;
-        s = t;
-      }
-    }
-    out[ch] = s + comp;
-  }
+  for (uint32_t ch = 0; ch < c.C; ++ch)
+    out[ch] = mass_combine(reinterpret_cast<const double2*>(part.data()) + (size_t)ch * e->mass_nblk);
   return CN_OK;
   CN_CATCH
 }
@@ -1215,7 +1297,7 @@ cn_status cn_creature_state_from_device(cn_engine* e, uint32_t i, const float* d
   CN_HIP(hipMemcpyAsync(cur_state(e, i), d_src, n * 4, hipMemcpyDeviceToDevice, e->stream));
   CN_HIP(hipStreamSynchronize(e->stream));
   e->have_prev = false;
-  e->ro_fwd_done = false;
+  e->res.fwd_valid = false;
   return CN_OK;
   CN_CATCH
 }
@@ -1267,8 +1349,13 @@ cn_status cn_rollout_grad(cn_engine* e, uint32_t steps, uint32_t ckpt_every) {
   cn_status st;
   if ((st = normalise_current(e)) != CN_OK) return st;
   if ((st = prepare_rollout(e, steps, ckpt_every)) != CN_OK) return st;
+  e->res.grads_valid = false;
+  e->res.loss_valid = false;
   CN_HIP(hipGraphLaunch(e->ro.graph, e->stream));
-  e->ro_fwd_done = true;
+  set_forward_result(e);
+  e->res.grad_slot = e->ro.grad_slot;
+  e->res.grads_valid = true;
+  e->res.loss_valid = true;
   return CN_OK;
   CN_CATCH
 }
@@ -1282,7 +1369,7 @@ cn_status cn_rollout_forward(cn_engine* e, uint32_t steps, uint32_t ckpt_every) 
   if ((st = normalise_current(e)) != CN_OK) return st;
   if ((st = prepare_rollout(e, steps, ckpt_every)) != CN_OK) return st;
   CN_HIP(hipGraphLaunch(e->ro.fwd, e->stream));
-  e->ro_fwd_done = true;
+  set_forward_result(e);
   return CN_OK;
   CN_CATCH
 }
@@ -1292,11 +1379,11 @@ cn_status cn_creature_rollout_final(cn_engine* e, uint32_t i, float* out, size_t
   clear_error();
   cn_status st = check_creature(e, i);
   if (st != CN_OK) return st;
-  if (!e->ro_fwd_done) return set_error(CN_ERR_STATE, "no rollout forward since the last state change");
+  if (!e->res.fwd_valid) return set_error(CN_ERR_STATE, "no rollout forward since the last state change");
   Creature& c = e->cr[i];
   if (!out || n != c.slot_elems()) return set_error(CN_ERR_INVALID_ARG, "bad output buffer");
   CN_HIP(hipSetDevice(e->device));
-  CN_HIP(hipMemcpyAsync(out, c.dev.state + (size_t)e->ro.final_slot * c.slot_elems(), n * 4,
+  CN_HIP(hipMemcpyAsync(out, c.dev.state + (size_t)e->res.final_slot * c.slot_elems(), n * 4,
                         device_pointer ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, e->stream));
   CN_HIP(hipStreamSynchronize(e->stream));
   return CN_OK;
@@ -1308,20 +1395,22 @@ cn_status cn_rollout_backward(cn_engine* e, const float* const* d_grad_final, ui
   clear_error();
   if (!e || !d_grad_final) return set_error(CN_ERR_INVALID_ARG, "null argument");
   if (count != e->cr.size()) return set_error(CN_ERR_INVALID_ARG, "expected %zu gradient pointers", e->cr.size());
-  if (!e->ro_fwd_done) return set_error(CN_ERR_STATE, "no rollout forward since the last state change");
+  if (!e->res.fwd_valid) return set_error(CN_ERR_STATE, "no rollout forward since the last state change");
   for (uint32_t i = 0; i < count; ++i)
     if (!d_grad_final[i]) return set_error(CN_ERR_INVALID_ARG, "null gradient pointer %u", i);
+  e->res.grads_valid = false;
   CN_HIP(hipSetDevice(e->device));
   for (uint32_t i = 0; i < count; ++i) {
     const Creature& c = e->cr[i];
     CN_HIP(hipMemcpyAsync(c.dev.grad, d_grad_final[i], c.slot_elems() * 4, hipMemcpyDeviceToDevice, e->stream));
   }
-  const SlotPlan p = plan_slots(e->ro.T, e->ro.ckpt);
+  const SlotPlan p = plan_slots(e->res.T, e->res.ckpt);
   int gslot = 0;
   // Other calls may have used intermediate slot 0 since the forward: recompute every segment.
-  CN_HIP(record_backward(e, p, e->ro.T, e->ro.ckpt, 0, false, &gslot));
+  CN_HIP(record_backward(e, p, e->res.T, e->res.ckpt, 0, false, &gslot));
   CN_HIP(hipStreamSynchronize(e->stream));
-  e->ro.grad_slot = gslot;
+  e->res.grad_slot = gslot;
+  e->res.grads_valid = true;
   return CN_OK;
   CN_CATCH
 }
@@ -1333,7 +1422,7 @@ cn_status cn_creature_get_loss(cn_engine* e, uint32_t i, double* loss) {
   if (st != CN_OK) return st;
   if (!loss) return set_error(CN_ERR_INVALID_ARG, "null argument");
   Creature& c = e->cr[i];
-  if (!c.adj) return set_error(CN_ERR_STATE, "no rollout has run");
+  if (!c.adj || !e->res.loss_valid) return set_error(CN_ERR_STATE, "no cn_rollout_grad has completed");
   CN_HIP(hipSetDevice(e->device));
   std::vector<float> part(kLossPartials);
   CN_HIP(hipMemcpyAsync(part.data(), c.dev.lossp, part.size() * 4, hipMemcpyDeviceToHost, e->stream));
@@ -1352,14 +1441,14 @@ cn_status cn_creature_get_grads(cn_engine* e, uint32_t i, float* grad_state0, si
   cn_status st = check_creature(e, i);
   if (st != CN_OK) return st;
   Creature& c = e->cr[i];
-  if (!c.adj) return set_error(CN_ERR_STATE, "no rollout backward has run");
+  if (!c.adj || !e->res.grads_valid) return set_error(CN_ERR_STATE, "no rollout backward has run");
   if (grad_state0 && n != c.slot_elems())
     return set_error(CN_ERR_INVALID_ARG, "grad_state0 must hold %zu values", c.slot_elems());
   if ((grad_mu || grad_sigma || grad_w) && K != c.K) return set_error(CN_ERR_INVALID_ARG, "expected K = %u", c.K);
   CN_HIP(hipSetDevice(e->device));
   const hipMemcpyKind kind = device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
   if (grad_state0)
-    CN_HIP(hipMemcpyAsync(grad_state0, c.dev.grad + (size_t)e->ro.grad_slot * c.slot_elems(), n * 4, kind,
+    CN_HIP(hipMemcpyAsync(grad_state0, c.dev.grad + (size_t)e->res.grad_slot * c.slot_elems(), n * 4, kind,
                           e->stream));
   float* outs[3] = {grad_mu, grad_sigma, grad_w};
   for (int j = 0; j < 3; ++j)
@@ -1478,7 +1567,7 @@ cn_status cn_autotune(cn_engine* e, uint32_t reps) {
   }
   drop_graphs(e);        // graphs bake the launch geometry in
   e->have_prev = false;  // the other slot was used as scratch
-  e->ro_fwd_done = false;
+  e->res.fwd_valid = false;
   return CN_OK;
   CN_CATCH
 }

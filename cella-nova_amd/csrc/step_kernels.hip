@@ -14,6 +14,7 @@
 #include "cfg.h"
 #include "fft.h"
 #include "launch.h"
+#include "mass.h"
 #include "realfft.h"
 #include "types.h"
 
@@ -397,43 +398,17 @@ k_diffusion(const CreatureDev* __restrict__ descs, int sin, int sout, int N, int
 }
 
 // ---------------------------------------------------------------------------------------
-// FP64 Neumaier-compensated mass per channel (the only FP64 arithmetic on the device).
+// FP64 Neumaier-compensated mass per channel (mass.h).
 // partial[(creature * 16 + c) * nblk + blockIdx.x] = (sum, compensation)
 // ---------------------------------------------------------------------------------------
-__device__ __forceinline__ void neumaier_add(double& s, double& comp, double x) {
-  const double t = s + x;
-  if (fabs(s) >= fabs(x)) comp += (s - t) + x;
-  else comp += (x - t) + s;
-  s = t;
-}
-
-constexpr int kMassThreads = 256;
-
 __global__ void __launch_bounds__(kMassThreads)
 k_mass(const CreatureDev* __restrict__ descs, int slot, double2* __restrict__ partial, int nblk) {
   const CreatureDev& d = descs[blockIdx.z];
   const int c = blockIdx.y;
-  double2* dst = partial + ((size_t)blockIdx.z * kMaxChannels + c) * nblk + blockIdx.x;
   if (c >= d.C) return;
   const size_t plane = (size_t)d.N * d.N;
-  gptr<const float> s = slot_ptr(d, slot) + c * plane;
-  double sum = 0.0, comp = 0.0;
-  for (size_t i = (size_t)blockIdx.x * kMassThreads + threadIdx.x; i < plane; i += (size_t)nblk * kMassThreads)
-    neumaier_add(sum, comp, (double)s[i]);
-  __shared__ double ss[kMassThreads], sc[kMassThreads];
-  ss[threadIdx.x] = sum;
-  sc[threadIdx.x] = comp;
-  __syncthreads();
-  for (int h = kMassThreads / 2; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) {
-      double a = ss[threadIdx.x], ca = sc[threadIdx.x] + sc[threadIdx.x + h];
-      neumaier_add(a, ca, ss[threadIdx.x + h]);
-      ss[threadIdx.x] = a;
-      sc[threadIdx.x] = ca;
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *dst = make_double2(ss[0], sc[0]);
+  mass_block(d.state + ((size_t)slot * d.C + c) * plane, plane, (int)blockIdx.x, nblk,
+             partial + ((size_t)blockIdx.z * kMaxChannels + c) * nblk + blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -195,6 +195,26 @@ class _Engine:
     def sync(self):
         check(lib().cn_sync(self._h))
 
+    def iterate_on_stream(self, stream):
+        """One step enqueued on a caller's HIP stream (torch.cuda.Stream, a raw handle, or None
+        for the null stream), ordered after the engine's queued work (iterate_with_encoder,
+        mod.rs:435-466)."""
+        handle = getattr(stream, "cuda_stream", stream)
+        check(lib().cn_iterate_on_stream(self._h, ctypes.c_void_p(handle or 0)))
+
+    def stream_handle(self):
+        """The engine's own hipStream_t as an integer (wrap with torch.cuda.ExternalStream)."""
+        h = ctypes.c_void_p()
+        check(lib().cn_engine_stream(self._h, ctypes.byref(h)))
+        return h.value or 0
+
+    def creature_state_ptr(self, i):
+        """(device address, element count) of creature i's current state (channel_buffer,
+        mod.rs:202-204); valid until the next step or rollout."""
+        p, n = ctypes.c_void_p(), ctypes.c_size_t()
+        check(lib().cn_creature_state_ptr(self._h, int(i), ctypes.byref(p), ctypes.byref(n)))
+        return p.value, n.value
+
     def rollout_grad(self, steps, ckpt_every):
         """Forward `steps` steps, MSE vs each creature's target, checkpointed adjoint (async)."""
         check(lib().cn_rollout_grad(self._h, int(steps), int(ckpt_every)))

@@ -74,6 +74,23 @@ cn_status cn_run_steps(cn_engine* e, uint64_t n);
 /* Wait for all queued work of the engine (device.poll(Wait) in the reference). */
 cn_status cn_sync(cn_engine* e);
 
+/* GpuMaceLenia::iterate_with_encoder (mod.rs:435-466), which records one step into the
+ * caller's command encoder: here one step is enqueued on the caller's HIP stream `stream`
+ * (a hipStream_t; NULL is the null stream) instead of the engine's.  The caller's stream first
+ * waits for the work already queued on the engine's stream, and the engine's stream waits for
+ * the step, so both orders hold without a host wait.  Asynchronous. */
+cn_status cn_iterate_on_stream(cn_engine* e, void* stream);
+/* New: the engine's own HIP stream (a hipStream_t), for callers that enqueue their work behind
+ * or ahead of the engine's with events.  Valid until cn_engine_destroy. */
+cn_status cn_engine_stream(const cn_engine* e, void** stream);
+/* GpuMaceLenia::channel_buffer (mod.rs:202-204): device address of creature i's current state,
+ * C*N*N f32, channel-major (*n receives C*N*N if n is not NULL).  The engine ping-pongs between
+ * two state buffers, so the address is valid until the next step or rollout call; work on
+ * other streams must be ordered with the engine's stream (cn_engine_stream).  Writing through
+ * it replaces the state like cn_creature_state_from_device, except that the engine is not told:
+ * cn_download_conv_real and cn_rollout_backward then refer to the state before the write. */
+cn_status cn_creature_state_ptr(cn_engine* e, uint32_t creature, float** d_state, size_t* n);
+
 /* GpuMaceLenia::timed_iterate (mod.rs:469-540): one step with per-phase device times in ms:
  * ms[0] convolution (row + fused column passes), ms[1] growth (inverse row pass fused with
  * the growth/affinity epilogue), ms[2] diffusion, ms[3] copy (0: ping-pong pointer swap). */

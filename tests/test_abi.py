@@ -173,3 +173,13 @@ def test_python_mirror_checks_tensor_arguments(cn):
     with pytest.raises(TypeError):
         _host_f32(np.zeros(5, np.float32), 4, "a")            # wrong size
     assert _host_f32(np.zeros(4, np.float32), 4, "a") != 0
+
+
+def test_stream_and_state_pointer_entries_validate(cn):
+    lib = cn.lib()
+    p = ctypes.c_void_p()
+    assert lib.cn_engine_stream(None, ctypes.byref(p)) == 1
+    assert lib.cn_iterate_on_stream(None, None) == 1
+    n = ctypes.c_size_t()
+    assert lib.cn_creature_state_ptr(None, 0, ctypes.byref(p), ctypes.byref(n)) == 1
+    assert len(lib.cn_last_error()) > 0
