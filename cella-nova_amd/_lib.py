@@ -83,6 +83,7 @@ SIGNATURES = {
     "cn_download_channels": (c_int, [c_void_p, c_void_p, c_size_t]),
     "cn_download_conv_real": (c_int, [c_void_p, c_void_p, c_size_t]),
     "cn_mass_fp64": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "cn_mass_fp64_noalloc": (c_int, [c_void_p, c_uint32, c_uint32, c_void_p, c_void_p]),
     "cn_iterate_and_render": (c_int, [c_void_p]),
     "cn_download_render": (c_int, [c_void_p, c_void_p, c_size_t]),
     "cn_creature_download_render": (c_int, [c_void_p, c_uint32, c_void_p, c_size_t]),

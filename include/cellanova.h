@@ -125,6 +125,18 @@ cn_status cn_creature_download_render(cn_engine* e, uint32_t i, uint8_t* rgb, si
  * device (out[c], c < C).  Synchronises. */
 cn_status cn_mass_fp64(cn_engine* e, double* out, size_t n_channels);
 
+/* New, with no counterpart in the reference: the FP64 mass of a caller's device state for
+ * callers that cannot allocate (code under HIP graph capture, serving loops that forbid
+ * allocation).  d_state: num_channels planes of n*n f32 in device memory (channel-major, like
+ * cn_creature_state_ptr's); d_out: num_channels doubles in device memory, the same
+ * Neumaier-compensated sums as cn_creature_mass_fp64 gives for that state, bit for bit.
+ * Enqueued on `stream` (a hipStream_t; NULL is the null stream) of the calling thread's current
+ * device; asynchronous.  Makes no allocation: the block partials live in a fixed device array
+ * of 16 KiB per device, so num_channels is at most 16 (n at most 65536), and calls that overlap
+ * on one device race on that array. */
+cn_status cn_mass_fp64_noalloc(const float* d_state, uint32_t n, uint32_t num_channels, double* d_out,
+                               void* stream);
+
 /* New: engine facts. */
 typedef struct cn_engine_info {
   uint32_t n, num_channels, num_kernels;
