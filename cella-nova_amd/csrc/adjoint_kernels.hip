@@ -146,8 +146,11 @@ k_growth_adj_row(const CreatureDev* __restrict__ descs, const float2* __restrict
   __shared__ float red[3][NT];
   using TF = TwLds<L, E, false>;
   __shared__ float2 twl[TF::size + SplitTw<N>::size];  // LDS twiddles (see k_row_fwd)
-  TF::fill(twl, tw, threadIdx.x, blockDim.x);
-  SplitTw<N>::fill(twl + TF::size, tw, threadIdx.x, blockDim.x);
+  {
+    TwFill<TF, SplitTw<N>, CF::THREADS> twf;
+    twf.load(tw, threadIdx.x);
+    twf.store(twl, threadIdx.x);
+  }
   red[0][threadIdx.x] = smu;
   red[1][threadIdx.x] = ssg;
   red[2][threadIdx.x] = sw;
@@ -187,8 +190,11 @@ k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) 
   using TF = TwLds<L, E, false>;
   using TI = TwLds<L, E, true>;
   __shared__ float2 twl[TF::size + TI::size + 1];  // both plans' twiddles (see TwLds)
-  TF::fill(twl, tw, threadIdx.x, blockDim.x);
-  TI::fill(twl + TF::size, tw, threadIdx.x, blockDim.x);
+  {
+    TwFill<TF, TI, TC * ColCfg<N, EC>::T> twf;
+    twf.load(tw, threadIdx.x);
+    twf.store(twl, threadIdx.x);
+  }
 
   // acc += v * conj(K^_k) at this thread's positions (compact real copy when available)
   auto accumulate = [&](int k, const float2* v, float2* acc) {
@@ -252,9 +258,10 @@ k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) 
     }
   }
   fft_run<L, E, TC, true, true, kTwN, 0, true>(acc, lds, t, lane, twl + TF::size);
+  // Result layout of the column passes (res_index), as the forward column pass writes uhat.
   gptr<float2> out = c64(d.spec) + (size_t)c * N * HALF;
 #pragma unroll
-  for (int i = 0; i < E; ++i) out[(size_t)out_pos<L, E, true>(t, i) * HALF + x] = acc[i];
+  for (int i = 0; i < E; ++i) out[res_index(N, out_pos<L, E, true>(t, i), x)] = acc[i];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -274,11 +281,14 @@ k_row_inv_add(const CreatureDev* __restrict__ descs, int gout, const float2* __r
   __shared__ float2 lds[LdsMap<L, ROWS, false>::size];
   using TI = TwLds<L, E, true>;
   __shared__ float2 twl[TI::size + SplitTw<N>::size];  // LDS twiddles (see k_row_fwd)
-  TI::fill(twl, tw, threadIdx.x, blockDim.x);
-  SplitTw<N>::fill(twl + TI::size, tw, threadIdx.x, blockDim.x);
+  {
+    TwFill<TI, SplitTw<N>, CF::THREADS> twf;
+    twf.load(tw, threadIdx.x);
+    twf.store(twl, threadIdx.x);
+  }
   __syncthreads();
   float2 v[E];
-  c2r_start<N, E, true>(c64((const void*)d.spec) + ((size_t)c * N + y) * L, t, twl + TI::size, v);
+  c2r_start_ct<N, E, true>((const float2*)d.spec + (size_t)c * N * L, y, t, twl + TI::size, v);
   fft_run<L, E, ROWS, false, true, kTwN, 0, true>(v, lds, t, lane, twl);
   const size_t row = ((size_t)c * N + y) * N;
   gptr<const float2> dd = c64((const void*)(d.dsd + row));

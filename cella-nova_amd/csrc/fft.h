@@ -212,25 +212,60 @@ struct TwLds {
   }
   static constexpr int size = off(PL::npass);
 
-  // Block-cooperative fill from the global table (layout in types.h); the caller's first
-  // exchange barrier orders it before any use.
-  template <int P = 1>
-  static __device__ __forceinline__ void fill(float2* dst, const float2* __restrict__ tw, int tid, int nth) {
-    if constexpr (P < PL::npass) {
+  // Index in the global table (layout in types.h) of LDS entry j, 0 <= j < size.
+  static __device__ __forceinline__ int src(int j) { return src_from<1>(j); }
+  template <int P>
+  static __device__ __forceinline__ int src_from(int j) {
+    if constexpr (P >= PL::npass) {
+      return 0;
+    } else {
       constexpr int r = R(P), ns = NS(P), m_n = r * ns;
-      const float2* tp = tw + tw_pass_offset(ns, r);
-      float2* d = dst + off(P);
-      if constexpr (direct(P)) {
-        for (int i = tid; i < (r - 1) * ns; i += nth) d[i] = tp[i];
-      } else {
-        for (int i = tid; i < (r - 1) * 16; i += nth) d[i] = tp[(i / 16) * ns + (i % 16)];
-        constexpr int nh = ns / 16;
-        for (int i = tid; i < (r - 1) * nh; i += nth) {
-          const int m = i / nh + 1, kh = i % nh;
-          d[(r - 1) * 16 + i] = tw[((16 * m * kh) * (kTwN / m_n)) % kTwN];
+      if (j < off(P) + pass_size(P)) {
+        const int i = j - off(P);
+        if constexpr (direct(P)) {
+          return tw_pass_offset(ns, r) + i;
+        } else {
+          constexpr int nh = ns / 16;
+          if (i < (r - 1) * 16) return tw_pass_offset(ns, r) + (i / 16) * ns + (i % 16);
+          const int ii = i - (r - 1) * 16, m = ii / nh + 1, kh = ii % nh;
+          return ((16 * m * kh) * (kTwN / m_n)) % kTwN;
         }
       }
-      fill<P + 1>(dst, tw, tid, nth);
+      return src_from<P + 1>(j);
+    }
+  }
+};
+
+// An empty LDS twiddle table (for TwFill with a single table).
+struct TwNone {
+  static constexpr int size = 0;
+  static __device__ __forceinline__ int src(int) { return 0; }
+};
+
+// Copies the LDS twiddle tables A then B (each with `size` entries and a `src` map into the
+// global table) into one LDS array with all of a thread's global loads issued before any LDS
+// store: one memory round trip per block instead of one per table part.  load() can be issued
+// before the kernel's other loads complete; store() waits for its own loads only, and the
+// caller's next barrier orders the stores before any use.  NTH: the block size.
+template <class A, class B, int NTH>
+struct TwFill {
+  static constexpr int S = A::size + B::size;
+  static constexpr int IT = S > 0 ? (S + NTH - 1) / NTH : 1;
+  float2 v[IT];
+  __device__ __forceinline__ void load(const float2* __restrict__ tw, int tid) {
+    gptr<const float2> g = to_global(tw);
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      int j = tid + it * NTH;
+      j = j < S ? j : 0;
+      v[it] = g[j < A::size ? A::src(j) : B::src(j - A::size)];
+    }
+  }
+  __device__ __forceinline__ void store(float2* dst, int tid) const {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int j = tid + it * NTH;
+      if (j < S) dst[j] = v[it];
     }
   }
 };

@@ -71,8 +71,11 @@ k_gen_rows(const CreatureDev* __restrict__ desc, const RingParams* __restrict__ 
   __shared__ float2 lds[LdsMap<L, ROWS, false>::size];
   using TF = TwLds<L, E, false>;
   __shared__ float2 twl[TF::size + SplitTw<N>::size];  // LDS twiddles (see k_row_fwd)
-  TF::fill(twl, tw, threadIdx.x, blockDim.x);
-  SplitTw<N>::fill(twl + TF::size, tw, threadIdx.x, blockDim.x);
+  {
+    TwFill<TF, SplitTw<N>, CF::THREADS> twf;
+    twf.load(tw, threadIdx.x);
+    twf.store(twl, threadIdx.x);
+  }
   __syncthreads();
   fft_run<L, E, ROWS, false, false, kTwN, 0, true>(v, lds, t, lane, twl);
   r2c_finish<N, E, ROWS, true>(v, lds, t, lane, twl + TF::size, c64(d.uhat) + (size_t)row * L,
@@ -96,7 +99,11 @@ k_gen_cols(const CreatureDev* __restrict__ desc, const float2* __restrict__ tw) 
   __shared__ float2 lds[LdsMap<L, TC, true>::size];
   using TF = TwLds<L, E, false>;
   __shared__ float2 twl[TF::size + 1];  // forward-plan twiddles (see TwLds)
-  TF::fill(twl, tw, threadIdx.x, blockDim.x);
+  {
+    TwFill<TF, TwNone, TC * ColCfg<N, EC>::T> twf;
+    twf.load(tw, threadIdx.x);
+    twf.store(twl, threadIdx.x);
+  }
   fft_run<L, E, TC, true, false, kTwN, 0, true>(v, lds, t, lane, twl);
 
   const float sc = 1.0f / ((float)N * (float)N);  // exact: N is a power of two

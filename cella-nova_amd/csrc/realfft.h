@@ -3,6 +3,7 @@
 // spectrum of a row is stored as N/2 complex values with slot 0 packing the two real bins
 // (X[0], X[N/2]) as (re, im).
 #pragma once
+#include "bufio.h"
 #include "fft.h"
 #include "types.h"
 
@@ -14,12 +15,7 @@ template <int N>
 struct SplitTw {
   static constexpr int NLO = 32, NHI = (N / 2 + 31) / 32;
   static constexpr int size = NLO + NHI;
-  static __device__ __forceinline__ void fill(float2* d, const float2* __restrict__ tw, int tid, int nth) {
-    for (int i = tid; i < size; i += nth) {
-      const int e = i < NLO ? i : 32 * (i - NLO);
-      d[i] = tw[(e * (kTwN / N)) % kTwN];
-    }
-  }
+  static __device__ __forceinline__ int src(int i) { return ((i < NLO ? i : 32 * (i - NLO)) * (kTwN / N)) % kTwN; }
   static __device__ __forceinline__ float2 get(const float2* d, int p) {
     return c_mul(d[NLO + (p >> 5)], d[p & 31]);
   }
@@ -111,6 +107,46 @@ __device__ __forceinline__ void c2r_load_own(gptr<const float2> X, int t, float2
   constexpr int L = N / 2;
 #pragma unroll
   for (int i = 0; i < E; ++i) xp[i] = X[in_pos<L, E, true>(t, i)];
+}
+
+// c2r_load_own for row y of a column-pass result plane (res_index).  Column-tiled: when the thread
+// count per row is a multiple of 4, element i of thread t is column t + c_i with c_i a multiple
+// of 4, so the tile (t >> 2) + c_i / 4 and the column inside it (t & 3) split into a per-thread
+// offset and a per-element constant.
+template <int N, int E>
+__device__ __forceinline__ void c2r_load_own_ct(const void* plane, int y, int t, float2* xp) {
+  constexpr int L = N / 2, T = L / (E < L ? E : L);
+  if constexpr (!kColTiled) {
+    const BufRes r = make_res(plane, (uint32_t)N * L * 8u);
+    const uint32_t vo = (uint32_t)(y * L + t) * 8u;
+#pragma unroll
+    for (int i = 0; i < E; ++i) xp[i] = buf_load_c64(r, vo, (uint32_t)in_pos<L, E, true>(0, i) * 8u);
+  } else if constexpr (T % 4 == 0) {
+    const BufRes r = make_res(plane, (uint32_t)N * L * 8u);
+    const uint32_t vo = (uint32_t)(((t >> 2) * N + y) * 4 + (t & 3)) * 8u;
+#pragma unroll
+    for (int i = 0; i < E; ++i) xp[i] = buf_load_c64(r, vo, (uint32_t)(in_pos<L, E, true>(0, i) >> 2) * N * 32u);
+  } else {
+    gptr<const float2> X = c64(plane);
+#pragma unroll
+    for (int i = 0; i < E; ++i) xp[i] = X[res_index(N, y, in_pos<L, E, true>(t, i))];
+  }
+}
+
+// c2r_start for row y of a column-pass result plane (res_index).
+template <int N, int E, bool LT = false>
+__device__ __forceinline__ void c2r_start_ct(const void* plane, int y, int t, const float2* __restrict__ tw,
+                                             float2* v) {
+  constexpr int L = N / 2;
+  gptr<const float2> X = c64(plane);
+  float2 xp[E], xq[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int p = in_pos<L, E, true>(t, i);
+    xp[i] = X[res_index(N, y, p)];
+    xq[i] = X[res_index(N, y, (L - p) & (L - 1))];
+  }
+  c2r_form<N, E, LT>(xp, xq, t, tw, v);
 }
 
 template <int N, int E, int ROWS>

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 #include "cfg.h"
 #include "fft.h"
+#include "bufio.h"
 #include "launch.h"
 #include "mass.h"
 #include "realfft.h"
@@ -47,8 +48,11 @@ k_row_fwd(const CreatureDev* __restrict__ descs, int sin, const float2* __restri
   // rows exchange at wavefront level, so the fill gets a block barrier of its own.
   using TF = TwLds<L, E, false>;
   __shared__ float2 twl[TF::size + SplitTw<N>::size];
-  TF::fill(twl, tw, threadIdx.x, blockDim.x);
-  SplitTw<N>::fill(twl + TF::size, tw, threadIdx.x, blockDim.x);
+  {
+    TwFill<TF, SplitTw<N>, CF::THREADS> twf;
+    twf.load(tw, threadIdx.x);
+    twf.store(twl, threadIdx.x);
+  }
   __syncthreads();
   __shared__ float2 lds[LdsMap<L, ROWS, false>::size];
   fft_run<L, E, ROWS, false, false, kTwN, 0, true>(v, lds, t, lane, twl);
@@ -63,11 +67,17 @@ k_row_fwd(const CreatureDev* __restrict__ descs, int sin, const float2* __restri
 // product with the kernel's half spectrum and inverse column FFT.  SPLIT: one block per
 // (tile, kernel) instead of per (tile, source), each redoing its source's forward transform —
 // K instead of C blocks per tile, for grids too small to fill the GPU by sources.
+//
+// Memory: the source spectrum is read from its row-major plane and the results are written in
+// the result layout (res_index: row-major, or column tiles whose 4 columns and 4 consecutive
+// rows form one 128-byte line).  Every load and store goes through a buffer resource of its
+// plane with a per-thread 32-bit offset and a compile-time per-element offset (bufio.h).
 // ---------------------------------------------------------------------------------------
 template <int N, int TC, int EC, bool SPLIT>
 __global__ void __launch_bounds__((TC * ColCfg<N, EC>::T)) __attribute__((amdgpu_waves_per_eu(4)))
 k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) {
   constexpr int L = N, E = ColCfg<N, EC>::E, HALF = N / 2;
+  constexpr uint32_t kPlaneBytes = (uint32_t)N * HALF * 8, kRealBytes = (uint32_t)(HALF + 1) * HALF * 4;
   const CreatureDev& d = descs[blockIdx.z];
   if (d.N != N) return;
   int c, kb, ke;
@@ -89,39 +99,60 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
   int t = threadIdx.x / TC;
   const int x = x0 + lane;
 
-  gptr<const float2> sp = c64((const void*)d.spec) + (size_t)c * N * HALF;
+  // Element i of a thread sits at row t + (its position at t = 0): one VGPR offset per thread
+  // (row t, column x) and one constant per element.
+  const BufRes sp = make_res((const char*)d.spec + (size_t)c * kPlaneBytes, kPlaneBytes);
   float2 v[E];
+  {
+    const uint32_t vo = (uint32_t)(t * HALF + x) * 8u;
 #pragma unroll
-  for (int i = 0; i < E; ++i) v[i] = sp[(size_t)in_pos<L, E, false>(t, i) * HALF + x];
+    for (int i = 0; i < E; ++i) v[i] = buf_load_c64(sp, vo, (uint32_t)in_pos<L, E, false>(0, i) * HALF * 8u);
+  }
 
   // Kernel spectra are prefetched one kernel ahead: the loads of kernel k+1's tile are in
   // flight while kernel k's inverse transform runs (the first ones during the forward one).
   // Kernels with a real spectrum that is even in ky are read from the compact real copy
-  // (a quarter of the bytes); the product is the same.
-  gptr<const float2> khalf = c64(d.khalf);
+  // (a quarter of the bytes) and enter a real product (2 multiplies per value instead of 6).
+  // Rows y > N/2 of a compact spectrum read row N - y; every element's side of N/2 is fixed
+  // at compile time (element offsets are multiples of N/E, as is N/2).
   auto load_kernel = [&](int k, float2* out) {
     if (d.ksym[k]) {
-      gptr<const float> kr = to_global(d.kreal) + (size_t)k * (HALF + 1) * HALF;
+      const BufRes kr = make_res(d.kreal + (size_t)k * (HALF + 1) * HALF, kRealBytes);
+      const uint32_t vb = (uint32_t)((x >> 2) * (HALF + 1) * 4 + (x & 3)) * 4u;
+      const uint32_t vp = vb + (uint32_t)t * 16u, vn = vb + (uint32_t)(N - t) * 16u;
 #pragma unroll
       for (int i = 0; i < E; ++i) {
-        const int y = out_pos<L, E, false>(t, i);
-        out[i] = make_float2(kr[kreal_index(N, y <= HALF ? y : N - y, x)], 0.0f);
+        const int ci = out_pos<L, E, false>(0, i);
+        out[i].x = ci < HALF ? buf_load_f32(kr, vp, (uint32_t)ci * 16u) : buf_load_f32(kr, vn - (uint32_t)ci * 16u, 0);
       }
     } else {
-      gptr<const float2> kh = khalf + (size_t)k * N * HALF;
+      const BufRes kh = make_res((const char*)d.khalf + (size_t)k * kPlaneBytes, kPlaneBytes);
+      const uint32_t vo = (uint32_t)(t * HALF + x) * 8u;
 #pragma unroll
-      for (int i = 0; i < E; ++i) out[i] = kh[(size_t)out_pos<L, E, false>(t, i) * HALF + x];
+      for (int i = 0; i < E; ++i) out[i] = buf_load_c64(kh, vo, (uint32_t)out_pos<L, E, false>(0, i) * HALF * 8u);
     }
   };
-  float2 nk[E];
-  load_kernel(kernel_of(kb), nk);
-  __shared__ float2 lds[LdsMap<L, TC, true>::size];
-  // Twiddles of both plans in LDS (see TwLds); the first exchange barrier orders the fill.
+  auto product = [&](int k, const float2* nkv, float2* wv) {
+    if (d.ksym[k]) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) wv[i] = make_float2(v[i].x * nkv[i].x, v[i].y * nkv[i].x);
+    } else {
+#pragma unroll
+      for (int i = 0; i < E; ++i) wv[i] = c_mul(v[i], nkv[i]);
+    }
+  };
+  // Twiddles of both plans in LDS (see TwLds): their loads go out right behind the spectrum's
+  // and ahead of the first kernel's, the LDS stores wait for them alone, and the first
+  // exchange barrier orders the stores.
   using TF = TwLds<L, E, false>;
   using TI = TwLds<L, E, true>;
   __shared__ float2 twl[TF::size + TI::size + 1];
-  TF::fill(twl, tw, threadIdx.x, blockDim.x);
-  TI::fill(twl + TF::size, tw, threadIdx.x, blockDim.x);
+  TwFill<TF, TI, TC * ColCfg<N, EC>::T> twf;
+  twf.load(tw, threadIdx.x);
+  float2 nk[E];
+  load_kernel(kernel_of(kb), nk);
+  twf.store(twl, threadIdx.x);
+  __shared__ float2 lds[LdsMap<L, TC, true>::size];
   fft_run<L, E, TC, true, false, kTwN, 0, true>(v, lds, t, lane, twl);
 
   // Tile 0, column 0 carries two real columns packed as (DC column) + i (Nyquist column).
@@ -136,15 +167,14 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
   // operation, and would wait for everything.
   const bool edge = (x0 == 0);
   float2 w[E];
-#pragma unroll
-  for (int i = 0; i < E; ++i) w[i] = c_mul(v[i], nk[i]);
+  product(kernel_of(kb), nk, w);
+  // Result plane offsets (res_index): row t of column x, and each element's row offset.
+  const uint32_t vct = kColTiled ? (uint32_t)(((x >> 2) * N + t) * 4 + (x & 3)) * 8u : (uint32_t)(t * HALF + x) * 8u;
+  constexpr uint32_t kRowStride = kColTiled ? 32u : (uint32_t)HALF * 8u;
   for (int kk = kb; kk < ke; ++kk) {
     const int k = kernel_of(kk);
     t = launder(t);
-    if (kk > kb) {
-#pragma unroll
-      for (int i = 0; i < E; ++i) w[i] = c_mul(v[i], nk[i]);
-    }
+    if (kk > kb) product(k, nk, w);
     if (kk + 1 < ke) load_kernel(kernel_of(kk + 1), nk);
     if (edge) {
       // Lane 0 splits the packed DC/Nyquist column; its kernel factors (khalf column 0, knyq)
@@ -156,7 +186,7 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
       }
       __syncthreads();
       if (lane == 0) {
-        gptr<const float2> kh = khalf + (size_t)k * N * HALF;
+        gptr<const float2> kh = c64(d.khalf) + (size_t)k * N * HALF;
         gptr<const float2> kn = c64(d.knyq) + (size_t)k * N;
         constexpr int G4 = 2;
 #pragma unroll
@@ -186,13 +216,13 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
     }
     // The pass is bound by its store stream: from the inverse transform on, the wave runs at
     // raised priority, so that waves with stores to issue are not held back by the others'
-    // loads and forward work (4 % on the pass at 2048^2; raising it around the spectrum loads
-    // instead made the pass slower).
+    // loads and forward work (raising it around the spectrum loads instead made the pass
+    // slower).
     __builtin_amdgcn_s_setprio(1);
     fft_run<L, E, TC, true, true, kTwN, 0, true>(w, lds, t, lane, twl + TF::size);
-    gptr<float2> uo = c64(d.uhat) + (size_t)k * N * HALF;
+    const BufRes uo = make_res((const char*)d.uhat + (size_t)k * kPlaneBytes, kPlaneBytes);
 #pragma unroll
-    for (int i = 0; i < E; ++i) uo[(size_t)out_pos<L, E, true>(t, i) * HALF + x] = w[i];
+    for (int i = 0; i < E; ++i) buf_store_c64(w[i], uo, vct, (uint32_t)out_pos<L, E, true>(0, i) * kRowStride);
     __builtin_amdgcn_s_setprio(0);
   }
 }
@@ -214,8 +244,11 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
   __shared__ float2 lds[LdsMap<L, ROWS, false>::size];
   using TI = TwLds<L, E, true>;
   __shared__ float2 twl[TI::size + SplitTw<N>::size];  // LDS twiddles, as in k_row_fwd
-  TI::fill(twl, tw, threadIdx.x, blockDim.x);
-  SplitTw<N>::fill(twl + TI::size, tw, threadIdx.x, blockDim.x);
+  {
+    TwFill<TI, SplitTw<N>, CF::THREADS> twf;
+    twf.load(tw, threadIdx.x);
+    twf.store(twl, threadIdx.x);
+  }
   __syncthreads();
   const float temp = d.temp;
   {
@@ -224,7 +257,8 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
     for (int i = 0; i < E; ++i) acc[i] = make_float2(0.f, 0.f);
     float wsum = 0.f;
     const int kb = d.dst_off[c], ke = d.dst_off[c + 1];
-    gptr<const float2> uhat = c64((const void*)d.uhat);
+    // Convolution results in the column pass's result layout (res_index).
+    auto uhat_plane = [&](int k) { return (const void*)((const float2*)d.uhat + (size_t)k * N * L); };
     // Each kernel's row is prefetched one kernel ahead (loads in flight during the transform).
     // Every lane loads only its own elements and takes the partners through LDS.  With 16
     // values per thread the prefetch registers would spill; the row is loaded at the start of
@@ -232,7 +266,7 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
     constexpr bool kWave = wave_rows<L, E>();
     constexpr bool kPrefetch = !(kWave && E >= 16);
     float2 xp[E], xq[E];
-    if (kPrefetch && kb < ke) c2r_load_own<N, E>(uhat + ((size_t)d.dst_k[kb] * N + y) * L, t0, xp);
+    if (kPrefetch && kb < ke) c2r_load_own_ct<N, E>(uhat_plane(d.dst_k[kb]), y, t0, xp);
     for (int kk = kb; kk < ke; ++kk) {
       const int k = d.dst_k[kk];
       const int t = launder(t0);
@@ -248,12 +282,12 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
       // Row loads at raised priority: without a prefetch they gate this wave's progress, and
       // issued ahead of the other waves' transform work they overlap it (2 % on the pass).
       __builtin_amdgcn_s_setprio(1);
-      if constexpr (!kPrefetch) c2r_load_own<N, E>(uhat + ((size_t)k * N + y) * L, t, xp);
+      if constexpr (!kPrefetch) c2r_load_own_ct<N, E>(uhat_plane(k), y, t, xp);
       __builtin_amdgcn_s_setprio(0);
       c2r_partner<N, E, ROWS>(xp, lds, t, lane, xq);
       c2r_form<N, E, true>(xp, xq, t, twl + TI::size, v);
       __builtin_amdgcn_s_setprio(1);
-      if (kPrefetch && kk + 1 < ke) c2r_load_own<N, E>(uhat + ((size_t)d.dst_k[kk + 1] * N + y) * L, t, xp);
+      if (kPrefetch && kk + 1 < ke) c2r_load_own_ct<N, E>(uhat_plane(d.dst_k[kk + 1]), y, t, xp);
       __builtin_amdgcn_s_setprio(0);
       fft_run<L, E, ROWS, false, true, kTwN, 0, true>(v, lds, t, lane, twl);
 #pragma unroll

@@ -4,7 +4,8 @@
 //   state   [nslots][C][N][N] f32  world-state slots: 0/1 ping-pong current state (the
 //                                  reference's channel/new_channel), the rest are rollout
 //                                  checkpoints and recompute buffers.  Kernels take slot indices.
-//   spec    [C][N][N/2]  c64   row-R2C spectra of a state (also reused by the adjoint)
+//   spec    [C][N][N/2]  c64   row-R2C spectra of a state (row-major); the adjoint's column pass
+//                              reuses it for its output, laid out like uhat (res_index)
 //   khalf   [K][N][N/2]  c64   Hermitian part of each kernel spectrum, pre-scaled by 1/N^2,
 //                              columns kx = 0 .. N/2-1
 //   knyq    [K][N]       c64   the kx = N/2 (Nyquist) column of the same spectra
@@ -13,7 +14,8 @@
 //                              columns (kreal_index) so that column tiles read contiguous runs;
 //                              ksym[k] = 1
 //   uhat    [K][N][N/2]  c64   row spectra of every convolution result (after the inverse
-//                              column pass); slot kx = 0 packs (DC, Nyquist) as (re, im)
+//                              column pass), row-major (res_index; column tiles ct_index when
+//                              kColTiled); slot kx = 0 packs (DC, Nyquist) as (re, im)
 //   A       [islots][C][N][N] f32 affinity exp(temp * sum_k w_k G_k) per destination channel;
 //                              slot 0 for stepping, one slot per step of a rollout segment
 //   u       [islots][K][N][N] f32 convolution results (readback and adjoint only)
@@ -94,6 +96,32 @@ struct CreatureDev {
   float w[kMaxKernels];
   int ksym[kMaxKernels];       // 1: kernel k is read from kreal (real, even in ky)
 };
+
+// Offset (in complex elements) of row y, column x of a column-tiled half-spectrum plane
+// (N rows, N/2 columns): the plane is N/8 tiles of 4 columns, each tile N rows of 4 values.
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+inline size_t ct_index(int N, int y, int x) {
+  return ((size_t)(x >> 2) * N + y) * 4 + (x & 3);
+}
+
+// Layout of the column pass's result planes (uhat, and the adjoint column pass's output in
+// spec): column tiles (ct_index) when kColTiled, else row-major rows like the other planes.
+// Row-major is the default: column tiles let the column pass write whole 128-byte lines (its
+// launch 84.5 -> 79.0 us at 2048^2 C3K9) but give the inverse-row pass 16 partial lines per
+// load instruction (52.3 -> 61.0 us), so the step is faster row-major (0.1797 against
+// 0.1814 ms per step, same run).
+#ifndef CN_COL_TILED
+#define CN_COL_TILED 0
+#endif
+constexpr bool kColTiled = CN_COL_TILED != 0;
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+inline size_t res_index(int N, int y, int x) {
+  return kColTiled ? ct_index(N, y, x) : (size_t)y * (N / 2) + x;
+}
 
 // Offset of (ky <= N/2, kx < N/2) in one kernel's kreal block: 4-column tiles, rows inside.
 #ifdef __HIPCC__
