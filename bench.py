@@ -153,11 +153,7 @@ def batched_grad(pkg, count=32, n=512, C=5, T=32, ckpt=4, iters=50, plain_iters=
     second seeded creature reaches after T steps, Adam on mu/sigma/weights.  Rate in
     grad-steps/s (one creature advanced and back-propagated through one step), batched (one
     engine, one graph per iteration for all creatures) vs the plain per-creature loop."""
-    rng = np.random.default_rng(seed)
-    radii = rng.integers(10, 19, size=count)
-    crs = [pkg.baseline_creature(n, num_channels=C, radius=int(radii[i]), seed=seed, creature=i) for i in range(count)]
-    tgs = [pkg.baseline_creature(n, num_channels=C, radius=int(radii[i]), seed=seed, creature=1000 + i)
-           for i in range(count)]
+    crs, tgs = pkg.creatures.search_batch_creatures(count, n, C, seed)
     tb = pkg.CreatureBatch(tgs)
     for i, cr in enumerate(tgs):
         tb.load(i, cr)
@@ -223,12 +219,8 @@ def batched_forward(pkg, steps=200, seed=0):
     K = C^2, radii U{8..20}, one engine and one graph per step for all of them.  The creatures
     are generated on the device (kernel spectra, growth parameters, seed); the host generator's
     time for the same creatures is sampled on a few of them for comparison."""
-    rng = np.random.default_rng(seed)
-    sizes = [256] * 85 + [512] * 85 + [1024] * 86
-    specs = []
-    for i, n in enumerate(sizes):
-        C = int(rng.integers(2, 5))
-        specs.append(pkg.creature_spec(n, C, int(rng.integers(8, 21)), seed=seed, creature=i))
+    specs = pkg.creatures.uneven_batch_specs(seed)
+    sizes = [sp["n"] for sp in specs]
     t0 = time.perf_counter()
     b = pkg.CreatureBatch.generated(specs)
     b.sync()

@@ -60,6 +60,11 @@ class CreatureParams(ctypes.Structure):
     ]
 
 
+class AdamParams(ctypes.Structure):
+    _fields_ = [("lr", c_float), ("beta1", c_float), ("beta2", c_float), ("eps", c_float),
+                ("mu_min", c_float), ("mu_max", c_float), ("sigma_min", c_float), ("weight_min", c_float)]
+
+
 class Plan(ctypes.Structure):
     _fields_ = [("n", c_uint32), ("fwd_e", c_uint32), ("inv_e", c_uint32), ("col_e", c_uint32),
                 ("col_tc", c_uint32), ("diff_th", c_uint32), ("col_split", c_uint32)]
@@ -100,6 +105,12 @@ SIGNATURES = {
                                      c_void_p, c_void_p]),
     "cn_debug_fail_next_hip": (None, [c_int]),
     "cn_debug_fail_hip_at": (None, [c_int]),
+    # creature search on the device
+    "cn_batch_get_growth": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "cn_batch_set_growth": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "cn_batch_get_param_grads": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "cn_adam_step": (c_int, [c_void_p, POINTER(AdamParams)]),
+    "cn_adam_reset": (c_int, [c_void_p]),
     # plans / autotuner
     "cn_autotune": (c_int, [c_void_p, c_uint32]),
     "cn_engine_size_classes": (c_uint32, [c_void_p]),

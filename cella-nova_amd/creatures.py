@@ -86,3 +86,29 @@ def creature_spec(n, num_channels, radius, patch=128, seed=0, creature=0, src=No
             "src": np.asarray(src, np.uint32), "dst": np.asarray(dst, np.uint32),
             "radius": np.broadcast_to(np.asarray(radius, np.uint32), (K,)).copy(), "patch": int(patch),
             "seed": int(seed), "creature": int(creature)}
+
+
+def uneven_batch_specs(seed=0, sizes=None):
+    """The uneven creature batch: 85 creatures at 256^2, 85 at 512^2 and 86 at 1024^2 (in that
+    order), channels C ~ U{2,3,4} with K = C^2 (all src x dst pairs), radius R ~ U{8..20}, drawn in
+    creature order from numpy's default_rng(seed); creature i uses generator keys (seed, i)."""
+    rng = np.random.default_rng(seed)
+    sizes = [256] * 85 + [512] * 85 + [1024] * 86 if sizes is None else list(sizes)
+    specs = []
+    for i, n in enumerate(sizes):
+        C = int(rng.integers(2, 5))
+        specs.append(creature_spec(n, C, int(rng.integers(8, 21)), seed=seed, creature=i))
+    return specs
+
+
+def search_batch_creatures(count=32, n=512, num_channels=5, seed=0):
+    """The batched-search creatures: `count` creatures of n^2, C channels, K = C^2 kernels, radius
+    R ~ U{10..18} (numpy default_rng(seed)), seed patch 128^2, and their targets: creature
+    1000 + i of the same distribution (to be run forward T steps by the caller)."""
+    rng = np.random.default_rng(seed)
+    radii = rng.integers(10, 19, size=count)
+    crs = [baseline_creature(n, num_channels=num_channels, radius=int(radii[i]), seed=seed, creature=i)
+           for i in range(count)]
+    tgs = [baseline_creature(n, num_channels=num_channels, radius=int(radii[i]), seed=seed, creature=1000 + i)
+           for i in range(count)]
+    return crs, tgs

@@ -248,6 +248,11 @@ struct cn_engine {
   hipEvent_t join[2] = {nullptr, nullptr};  // cross-stream ordering (cn_iterate_on_stream)
   Rollout ro;               // last captured rollout plan
   RolloutResult res;        // results of the last rollout forward / backward
+  // cn_adam_step writes (mu, sigma, inv_s2, w) into the device descriptors: until they are read
+  // back, the host copies of those fields are stale (refresh_host_growth).
+  bool growth_on_device = false;
+  size_t nparams = 0;       // sum over creatures of 3K (batch-wide parameter arrays)
+  float* grad_gather = nullptr;  // device staging of every creature's parameter gradients
 };
 
 namespace {
@@ -302,7 +307,32 @@ void release(cn_engine* e) {
   delete e;
 }
 
+// Reads the growth parameters an optimiser step left in the device descriptors back into the
+// host copies (one copy per size class).  Synchronises.
+cn_status refresh_host_growth(cn_engine* e) {
+  if (!e->growth_on_device) return CN_OK;
+  for (const SizeClass& k : e->cls) {
+    std::vector<CreatureDev> tmp(k.members.size());
+    CN_HIP(hipMemcpyAsync(tmp.data(), k.d_descs, tmp.size() * sizeof(CreatureDev), hipMemcpyDeviceToHost, e->stream));
+    CN_HIP(hipStreamSynchronize(e->stream));
+    for (size_t j = 0; j < k.members.size(); ++j) {
+      CreatureDev& h = e->cr[k.members[j]].dev;
+      const CreatureDev& g = tmp[j];
+      std::memcpy(h.mu, g.mu, sizeof h.mu);
+      std::memcpy(h.sigma, g.sigma, sizeof h.sigma);
+      std::memcpy(h.inv_s2, g.inv_s2, sizeof h.inv_s2);
+      std::memcpy(h.w, g.w, sizeof h.w);
+    }
+  }
+  e->growth_on_device = false;
+  return CN_OK;
+}
+
 cn_status upload_desc(cn_engine* e, int i) {
+  // The host copy is about to overwrite the device descriptor: bring its growth fields up to
+  // date first.
+  const cn_status sr = refresh_host_growth(e);
+  if (sr != CN_OK) return sr;
   Creature& c = e->cr[i];
   CN_HIP(hipMemcpyAsync(e->cls[c.cls].d_descs + c.idx, &c.dev, sizeof(CreatureDev), hipMemcpyHostToDevice,
                         e->stream));
@@ -735,6 +765,8 @@ cn_status create_engine(const cn_engine_desc* descs, uint32_t count, int device,
     if ((st = dalloc(e, c.allocs, &p, C * P * 4)) != CN_OK) return fail(st);
     d.A = (float*)p;
     fill_desc_params(d, c.C, c.K, dd.src, dd.dst, dd.mu, dd.sigma, dd.weights, dd.temp);
+    d.param_ofs = (int)e->nparams;
+    e->nparams += 3 * (size_t)c.K;
   }
   for (SizeClass& k : e->cls) {
     void* p = nullptr;
@@ -890,8 +922,9 @@ cn_status cn_creature_set_growth(cn_engine* e, uint32_t i, const float* mu, cons
   if ((st = validate_growth(c.K, mu, sigma, w)) != CN_OK) return st;
   // A backward must recompute its segments with the parameters of its own forward.
   e->res.fwd_valid = false;
-  fill_growth(c.dev, c.K, mu, sigma, w);
   CN_HIP(hipSetDevice(e->device));
+  if ((st = refresh_host_growth(e)) != CN_OK) return st;  // before this creature's copy changes
+  fill_growth(c.dev, c.K, mu, sigma, w);
   return upload_desc(e, (int)i);
   CN_CATCH
 }
@@ -923,6 +956,7 @@ cn_status cn_creature_generate(cn_engine* e, uint32_t i, const cn_creature_param
     if ((st = dalloc(e, e->allocs, &q, kMaxKernels * sizeof(RingParams))) != CN_OK) return st;
     e->gen_rings = (RingParams*)q;
   }
+  if ((st = refresh_host_growth(e)) != CN_OK) return st;  // before this creature's copy changes
   fill_growth(c.dev, c.K, mu.data(), sg.data(), w.data());
   // A periodic kernel that depends on dx^2 + dy^2 only is even in x and y, so its spectrum is
   // real and even: every generated kernel uses the compact real copy.
@@ -1454,6 +1488,153 @@ cn_status cn_creature_get_grads(cn_engine* e, uint32_t i, float* grad_state0, si
   for (int j = 0; j < 3; ++j)
     if (outs[j]) CN_HIP(hipMemcpyAsync(outs[j], c.dev.gparam + (size_t)j * c.K, c.K * 4, kind, e->stream));
   CN_HIP(hipStreamSynchronize(e->stream));
+  return CN_OK;
+  CN_CATCH
+}
+
+// ---------------------------------------------------------------------------------------
+// batch-wide parameter copies and the device-side optimiser step
+// ---------------------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+// Uploads every size class's descriptor array from the host copies (one copy per class).
+cn_status upload_all_descs(cn_engine* e) {
+  for (const SizeClass& k : e->cls) {
+    std::vector<CreatureDev> tmp(k.members.size());
+    for (size_t j = 0; j < k.members.size(); ++j) tmp[j] = e->cr[k.members[j]].dev;
+    CN_HIP(hipMemcpyAsync(k.d_descs, tmp.data(), tmp.size() * sizeof(CreatureDev), hipMemcpyHostToDevice, e->stream));
+    CN_HIP(hipStreamSynchronize(e->stream));
+  }
+  return CN_OK;
+}
+
+// Adam state (zeroed moments) of every creature, allocated on first use.
+cn_status ensure_adam(cn_engine* e) {
+  bool grown = false;
+  for (auto& c : e->cr) {
+    if (c.dev.adam) continue;
+    void* p = nullptr;
+    const cn_status st = dalloc(e, c.allocs, &p, (6 * (size_t)c.K + 2) * 4);
+    if (st != CN_OK) return st;
+    c.dev.adam = (float*)p;
+    grown = true;
+  }
+  if (!grown) return CN_OK;
+  cn_status st = refresh_host_growth(e);
+  if (st != CN_OK) return st;
+  if ((st = upload_all_descs(e)) != CN_OK) return st;
+  for (const SizeClass& k : e->cls)
+    CN_HIP(launch_adam_reset(k.d_descs, (int)k.members.size(), e->stream));
+  return CN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+cn_status cn_batch_get_growth(cn_engine* e, float* out, size_t n) {
+  CN_TRY
+  clear_error();
+  if (!e || !out) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  if (n != e->nparams) return set_error(CN_ERR_INVALID_ARG, "expected %zu values (3K per creature)", e->nparams);
+  CN_HIP(hipSetDevice(e->device));
+  const cn_status st = refresh_host_growth(e);
+  if (st != CN_OK) return st;
+  for (const auto& c : e->cr) {
+    float* o = out + c.dev.param_ofs;
+    std::memcpy(o, c.dev.mu, c.K * 4);
+    std::memcpy(o + c.K, c.dev.sigma, c.K * 4);
+    std::memcpy(o + 2 * c.K, c.dev.w, c.K * 4);
+  }
+  return CN_OK;
+  CN_CATCH
+}
+
+cn_status cn_batch_set_growth(cn_engine* e, const float* params, size_t n) {
+  CN_TRY
+  clear_error();
+  if (!e || !params) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  if (n != e->nparams) return set_error(CN_ERR_INVALID_ARG, "expected %zu values (3K per creature)", e->nparams);
+  for (size_t i = 0; i < e->cr.size(); ++i) {
+    const auto& c = e->cr[i];
+    const float* q = params + c.dev.param_ofs;
+    const cn_status st = validate_growth(c.K, q, q + c.K, q + 2 * c.K);
+    if (st != CN_OK) {
+      char msg[kErrorTextSize];
+      std::memcpy(msg, last_error_text(), sizeof msg);
+      return set_error(st, "creature %zu: %s", i, msg);
+    }
+  }
+  CN_HIP(hipSetDevice(e->device));
+  cn_status st = refresh_host_growth(e);  // the device copies are about to be replaced anyway
+  if (st != CN_OK) return st;
+  for (auto& c : e->cr) {
+    const float* q = params + c.dev.param_ofs;
+    fill_growth(c.dev, c.K, q, q + c.K, q + 2 * c.K);
+  }
+  e->res.fwd_valid = false;
+  return upload_all_descs(e);
+  CN_CATCH
+}
+
+cn_status cn_batch_get_param_grads(cn_engine* e, float* out, size_t n) {
+  CN_TRY
+  clear_error();
+  if (!e || !out) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  if (n != e->nparams) return set_error(CN_ERR_INVALID_ARG, "expected %zu values (3K per creature)", e->nparams);
+  if (!e->res.grads_valid) return set_error(CN_ERR_STATE, "no rollout backward has run");
+  CN_HIP(hipSetDevice(e->device));
+  if (!e->grad_gather) {
+    void* p = nullptr;
+    const cn_status st = dalloc(e, e->allocs, &p, e->nparams * 4);
+    if (st != CN_OK) return st;
+    e->grad_gather = (float*)p;
+  }
+  for (const SizeClass& k : e->cls)
+    CN_HIP(launch_gather_grads(k.d_descs, (int)k.members.size(), e->grad_gather, e->stream));
+  CN_HIP(hipMemcpyAsync(out, e->grad_gather, n * 4, hipMemcpyDeviceToHost, e->stream));
+  CN_HIP(hipStreamSynchronize(e->stream));
+  return CN_OK;
+  CN_CATCH
+}
+
+cn_status cn_adam_reset(cn_engine* e) {
+  CN_TRY
+  clear_error();
+  if (!e) return set_error(CN_ERR_INVALID_ARG, "null engine");
+  CN_HIP(hipSetDevice(e->device));
+  const bool had = !e->cr.empty() && e->cr[0].dev.adam;
+  const cn_status st = ensure_adam(e);  // a fresh allocation starts reset
+  if (st != CN_OK) return st;
+  if (had)
+    for (const SizeClass& k : e->cls)
+      CN_HIP(launch_adam_reset(k.d_descs, (int)k.members.size(), e->stream));
+  return CN_OK;
+  CN_CATCH
+}
+
+cn_status cn_adam_step(cn_engine* e, const cn_adam_params* p) {
+  CN_TRY
+  clear_error();
+  if (!e || !p) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  const float v[8] = {p->lr, p->beta1, p->beta2, p->eps, p->mu_min, p->mu_max, p->sigma_min, p->weight_min};
+  for (float x : v)
+    if (!std::isfinite(x)) return set_error(CN_ERR_INVALID_ARG, "Adam parameters must be finite");
+  if (!(p->lr > 0.f) || !(p->eps > 0.f)) return set_error(CN_ERR_INVALID_ARG, "lr and eps must be > 0");
+  if (!(p->beta1 >= 0.f && p->beta1 < 1.f) || !(p->beta2 >= 0.f && p->beta2 < 1.f))
+    return set_error(CN_ERR_INVALID_ARG, "beta1 and beta2 must be in [0, 1)");
+  if (!(p->mu_min <= p->mu_max)) return set_error(CN_ERR_INVALID_ARG, "mu_min must be <= mu_max");
+  if (!(p->sigma_min > 0.f)) return set_error(CN_ERR_INVALID_ARG, "sigma_min must be > 0 (sigma stays positive)");
+  if (!e->res.grads_valid) return set_error(CN_ERR_STATE, "no rollout backward has run");
+  CN_HIP(hipSetDevice(e->device));
+  const cn_status st = ensure_adam(e);
+  if (st != CN_OK) return st;
+  const AdamParams ap{p->lr, p->beta1, p->beta2, p->eps, p->mu_min, p->mu_max, p->sigma_min, p->weight_min};
+  for (const SizeClass& k : e->cls) CN_HIP(launch_adam(k.d_descs, (int)k.members.size(), ap, e->stream));
+  e->growth_on_device = true;
+  e->res.fwd_valid = false;  // later backwards must not mix in the new parameters
   return CN_OK;
   CN_CATCH
 }

@@ -47,6 +47,17 @@ hipError_t launch_generate(int N, const CreatureDev* desc, RingParams* rings, in
 hipError_t launch_seed(const CreatureDev* desc, int slot, size_t cells, uint32_t side, uint32_t origin,
                        uint64_t seed, uint64_t creature, hipStream_t st);
 
+// Device-side Adam on every creature's (mu, sigma, w) from its last backward's gradients, then
+// the box mu in [mu_lo, mu_hi], sigma >= sigma_lo, w >= w_lo (writes the device descriptors).
+struct AdamParams {
+  float lr, beta1, beta2, eps;
+  float mu_lo, mu_hi, sigma_lo, w_lo;
+};
+hipError_t launch_adam(CreatureDev* descs, int count, const AdamParams& p, hipStream_t st);
+hipError_t launch_adam_reset(const CreatureDev* descs, int count, hipStream_t st);
+// The 3K parameter gradients of every creature into out[desc.param_ofs ...].
+hipError_t launch_gather_grads(const CreatureDev* descs, int count, float* out, hipStream_t st);
+
 hipError_t launch_mass(const CreatureDev* descs, int count, int slot, int max_c, double2* partial, int nblk,
                        hipStream_t st);
 

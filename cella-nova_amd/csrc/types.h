@@ -30,6 +30,8 @@
 //   gparam  [2][3][K]    f32   accumulated (mu, sigma, w) gradients and their Kahan terms
 //   target  [C][N][N]    f32   loss target
 //   lossp   [nblk]       f32   per-block loss partial sums
+//   adam    [6K + 2]     f32   Adam moments of (mu, sigma, w) and the bias-correction products
+//                              (allocated on the first cn_adam_step)
 #pragma once
 #include <stdint.h>
 
@@ -80,6 +82,8 @@ struct CreatureDev {
   float* gparam;
   float* target;
   float* lossp;
+  float* adam;         // [3K] first moments, [3K] second moments, beta1^t, beta2^t (optimiser)
+  int param_ofs;       // offset of this creature's 3K parameters in a batch-wide array
   int N, C, K;
   float temp;
   // kernels grouped by source channel (CSR): src_k[src_off[c] .. src_off[c+1]) in increasing k

@@ -5,7 +5,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import EngineDesc, EngineInfo, Plan, check, lib
+from ._lib import AdamParams, EngineDesc, EngineInfo, Plan, check, lib
 
 
 def _u32(a):
@@ -229,6 +229,37 @@ class _Engine:
         arr = (ctypes.c_void_p * len(grad_finals))(
             *[_dev_f32(g, self._elems(i), f"grad_finals[{i}]") for i, g in enumerate(grad_finals)])
         check(lib().cn_rollout_backward(self._h, arr, len(grad_finals)))
+
+    # --- creature search on the device ----------------------------------------------------
+    def _nparams(self):
+        return sum(3 * K for _, _, K in self._shapes)
+
+    def batch_growth(self):
+        """All creatures' (mu, sigma, w): a flat float32 array, creature i's [mu K | sigma K | w K]
+        after creature i-1's."""
+        out = np.empty(self._nparams(), np.float32)
+        check(lib().cn_batch_get_growth(self._h, out.ctypes.data, out.size))
+        return out
+
+    def batch_set_growth(self, params):
+        a = _f32(params).reshape(-1)
+        check(lib().cn_batch_set_growth(self._h, a.ctypes.data, a.size))
+
+    def batch_param_grads(self):
+        """dL/d(mu, sigma, w) of the last backward for all creatures, laid out as batch_growth."""
+        out = np.empty(self._nparams(), np.float32)
+        check(lib().cn_batch_get_param_grads(self._h, out.ctypes.data, out.size))
+        return out
+
+    def adam_step(self, lr=5e-3, beta1=0.9, beta2=0.999, eps=1e-8, mu_min=0.0, mu_max=2.0, sigma_min=1e-3,
+                  weight_min=0.0):
+        """Device-side Adam on every creature's (mu, sigma, w) from the last backward, then the
+        sanitize box of the reference's LeniaParams (asynchronous)."""
+        p = AdamParams(lr, beta1, beta2, eps, mu_min, mu_max, sigma_min, weight_min)
+        check(lib().cn_adam_step(self._h, ctypes.byref(p)))
+
+    def adam_reset(self):
+        check(lib().cn_adam_reset(self._h))
 
     def autotune(self, reps=5):
         """Pick the fastest FFT / stencil geometry per grid size on the device."""

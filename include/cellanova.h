@@ -205,6 +205,33 @@ cn_status cn_creature_get_grads(cn_engine* e, uint32_t creature, float* grad_sta
                                 float* grad_sigma, float* grad_w, size_t K, int device_pointers);
 
 /* ------------------------------------------------------------------------------------ */
+/* Creature search on the device (no reference counterpart: the reference runs one        */
+/* creature at a time through torch.optim.Adam, train/diff_lenia_org.py, and clamps the    */
+/* parameters with LeniaParams._sanitize, train/utils/leniaparams.py:351-360)            */
+/* ------------------------------------------------------------------------------------ */
+
+/* Batch-wide parameter arrays: creature i's 3K values [mu K | sigma K | weights K] follow
+ * those of creature i-1; n is the sum over creatures of 3K.  get/set copy all creatures at
+ * once (one synchronisation); get_param_grads returns dL/d(mu, sigma, w) of the last backward. */
+cn_status cn_batch_get_growth(cn_engine* e, float* out, size_t n);
+cn_status cn_batch_set_growth(cn_engine* e, const float* params, size_t n);
+cn_status cn_batch_get_param_grads(cn_engine* e, float* out, size_t n);
+
+/* Adam on every creature's (mu, sigma, w), on the device, from the gradients of the last
+ * rollout backward: m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g^2, step t counted per call,
+ * p -= lr (m / (1 - b1^t)) / (sqrt(v / (1 - b2^t)) + eps) in FP32, then mu is clamped to
+ * [mu_min, mu_max], sigma to >= sigma_min (> 0) and the weights to >= weight_min.  The new
+ * parameters are written where the step kernels read them: a loop of cn_rollout_grad and
+ * cn_adam_step runs with no host round trip.  Asynchronous.  The moments start at zero on
+ * the first call and after cn_adam_reset. */
+typedef struct cn_adam_params {
+  float lr, beta1, beta2, eps;
+  float mu_min, mu_max, sigma_min, weight_min;
+} cn_adam_params;
+cn_status cn_adam_step(cn_engine* e, const cn_adam_params* p);
+cn_status cn_adam_reset(cn_engine* e);
+
+/* ------------------------------------------------------------------------------------ */
 /* Launch plans and the autotuner (no reference counterpart: the reference hard-codes one */
 /* shader per grid size, mod.rs:30-39)                                                    */
 /* ------------------------------------------------------------------------------------ */

@@ -183,3 +183,16 @@ def test_stream_and_state_pointer_entries_validate(cn):
     n = ctypes.c_size_t()
     assert lib.cn_creature_state_ptr(None, 0, ctypes.byref(p), ctypes.byref(n)) == 1
     assert len(lib.cn_last_error()) > 0
+
+
+def test_adam_and_batch_parameter_entries_validate(cn):
+    from cella_nova_amd._lib import AdamParams
+    lib = cn.lib()
+    p = AdamParams(5e-3, 0.9, 0.999, 1e-8, 0.0, 2.0, 1e-3, 0.0)
+    assert lib.cn_adam_step(None, ctypes.byref(p)) == 1
+    assert lib.cn_adam_reset(None) == 1
+    buf = np.zeros(27, np.float32)
+    assert lib.cn_batch_get_growth(None, buf.ctypes.data, buf.size) == 1
+    assert lib.cn_batch_set_growth(None, buf.ctypes.data, buf.size) == 1
+    assert lib.cn_batch_get_param_grads(None, buf.ctypes.data, buf.size) == 1
+    assert len(lib.cn_last_error()) > 0
