@@ -147,13 +147,51 @@ def torch_reference_rate(crs, targets, T, iters, device="cuda:0"):
     return len(setups) * T * iters / dt, float(np.mean(losses))
 
 
-def batched_grad(pkg, count=32, n=512, C=5, T=32, ckpt=4, iters=50, plain_iters=None, seed=0, torch_iters=2):
+def grad_step_bytes_per_cell(C, K, T, ckpt):
+    """Algorithmic bytes per grad-step (one creature advanced and back-propagated through one
+    step) per grid site, for a T-step rollout with a checkpoint every `ckpt` steps, all kernel
+    spectra compact: the inputs and outputs of every launch of the rollout, divided by T.
+      forward step   28C + 9K (as the step), + 4K for each step of the last segment (writes u)
+      recompute step 28C + 13K (the other segments' steps again, writing u), minus the
+                     diffusion (12C) of each such segment's last step
+      adjoint step   diffusion adjoint 20C (A, state, g in; dsd, gaff out), growth adjoint with
+                     the row transform 12K (u, gaff in; spectrum out), column adjoint 4C + 5K,
+                     row inverse 12C (spectrum, dsd in; gradient out)
+      loss           12C once (final state and target in, gradient out)"""
+    ck = min(ckpt, T)
+    nck = (T + ck - 1) // ck
+    last = T - (nck - 1) * ck
+    fwd = T * (28 * C + 9 * K) + last * 4 * K
+    rec = (T - last) * (28 * C + 13 * K) - (nck - 1) * 12 * C
+    adj = T * (36 * C + 17 * K)
+    return (fwd + rec + adj + 12 * C) / T
+
+
+def _apply_plans(eng, plans):
+    """Launch plans (cn_plan dicts, e.g. the "plans" a tuned run printed) for the engine's size
+    classes, instead of tuning: keeps the tuner's candidate launches out of a profile."""
+    by_n = {int(p["n"]): p for p in plans}
+    for cur in eng.plans():
+        p = by_n.get(int(cur["n"]))
+        if p is not None:
+            eng.set_plan(**{k: int(v) for k, v in p.items()})
+    return eng.plans()
+
+
+def batched_grad(pkg, count=32, n=512, C=5, T=32, ckpt=4, iters=50, plain_iters=None, seed=0, torch_iters=2,
+                 plans=None):
     """Batched gradient-based creature search: `count` creatures (C channels, K = C^2 kernels,
     radii U{10..18}), T-step rollouts with a checkpoint every `ckpt` steps, MSE to the state a
-    second seeded creature reaches after T steps, Adam on mu/sigma/weights.  Rate in
-    grad-steps/s (one creature advanced and back-propagated through one step), batched (one
-    engine, one graph per iteration for all creatures) vs the plain per-creature loop."""
+    second seeded creature reaches after T steps, Adam on mu/sigma/weights with the sanitize box
+    of LeniaParams._sanitize.  Rate in grad-steps/s (one creature advanced and back-propagated
+    through one step):
+      batched  one engine for all creatures: per iteration one rollout graph and one device-side
+               Adam launch, no host round trip;
+      plain    one engine per creature (the same kernels, graphs and device-side Adam, one tuned
+               plan), iterated creature by creature;
+      torch    the reference's PyTorch path per creature (torch_reference_rate)."""
     crs, tgs = pkg.creatures.search_batch_creatures(count, n, C, seed)
+    K = C * C
     tb = pkg.CreatureBatch(tgs)
     for i, cr in enumerate(tgs):
         tb.load(i, cr)
@@ -161,52 +199,58 @@ def batched_grad(pkg, count=32, n=512, C=5, T=32, ckpt=4, iters=50, plain_iters=
     targets = [tb.creature_download(i) for i in range(count)]
     tb.close()
 
-    def run(engines, idx, n_iter):
-        params = [[crs[i]["mu"].astype(np.float64), crs[i]["sigma"].astype(np.float64),
-                   crs[i]["weights"].astype(np.float64)] for i in range(count)]
-        mom = [[np.zeros_like(p) for p in ps] for ps in params]
-        vel = [[np.zeros_like(p) for p in ps] for ps in params]
-        losses = []
-        t0 = None
-        for it in range(n_iter + 1):  # iteration 0 captures the graphs (untimed)
-            if it == 1:
-                t0 = time.perf_counter()
-            for eng, members in engines:
-                eng.rollout_grad(T, ckpt)
-                for j, i in enumerate(members):
-                    g = eng.creature_grads(j, with_state=False)
-                    new = [_adam(params[i][q], g[key].astype(np.float64), mom[i][q], vel[i][q], it + 1)
-                           for q, key in enumerate(("mu", "sigma", "weights"))]
-                    params[i] = list(_sanitize(*new))
-                    eng.creature_set_growth(j, *params[i])
-                if it == n_iter:
-                    losses += [eng.creature_loss(j) for j in range(len(members))]
-        return (time.perf_counter() - t0), losses
-
-    def build(members):
+    def build(members, plan=None):
         eng = pkg.CreatureBatch([crs[i] for i in members])
         for j, i in enumerate(members):
             eng.load(j, crs[i])
             eng.creature_set_target(j, targets[i])
-        eng.autotune(3)  # every engine, batched or per creature, runs its own tuned plan
-        return eng, members
+        if plan is None:
+            plan = _apply_plans(eng, plans)[0] if plans else eng.autotune(3)[0]
+        else:
+            eng.set_plan(**plan)
+        return eng, plan
 
-    batch = [build(list(range(count)))]
-    dt_b, loss_b = run(batch, None, iters)
-    for e, _ in batch:
-        e.close()
-    plain = [build([i]) for i in range(count)]
-    pi = plain_iters or iters
-    dt_p, loss_p = run(plain, None, pi)
-    for e, _ in plain:
-        e.close()
+    def run(engines, n_iter):
+        """engines: (engine, member creatures); returns (seconds, final losses)."""
+        for eng, members in engines:  # iteration 0 captures the graphs, allocates the Adam state
+            eng.rollout_grad(T, ckpt)
+            eng.adam_step()
+            eng.adam_reset()         # ... and the timed iterations start from the initial state
+            eng.batch_set_growth(np.concatenate([np.concatenate([crs[i][k] for k in ("mu", "sigma", "weights")])
+                                                 for i in members]))
+            eng.sync()
+        t0 = time.perf_counter()
+        for _ in range(n_iter):
+            for eng, _ in engines:
+                eng.rollout_grad(T, ckpt)
+                eng.adam_step()
+        for eng, _ in engines:
+            eng.sync()
+        dt = time.perf_counter() - t0
+        return dt, [eng.creature_loss(j) for eng, members in engines for j in range(len(members))]
+
+    batch, plan = build(list(range(count)))
+    dt_b, loss_b = run([(batch, list(range(count)))], iters)
+    batch.close()
+    pi = iters if plain_iters is None else plain_iters
+    rate_p, loss_p = None, [float("nan")]
+    if pi:
+        plain = [(build([i], plan)[0], [i]) for i in range(count)]
+        dt_p, loss_p = run(plain, pi)
+        for e, _ in plain:
+            e.close()
+        rate_p = count * T * pi / dt_p
     rate_b = count * T * iters / dt_b
-    rate_p = count * T * pi / dt_p
-    out = {"config": f"{count} creatures {n}x{n} C{C}K{C * C} R~U{{10..18}}, T={T}, ckpt {ckpt}, Adam on "
-                     f"mu/sigma/w, {iters} iterations (plain loop: {pi}, PyTorch reference path: {torch_iters})",
-           "grad_steps_per_s_batched": round(rate_b, 1), "grad_steps_per_s_plain": round(rate_p, 1),
-           "ratio": round(rate_b / rate_p, 2), "final_mse_mean_batched": float(np.mean(loss_b)),
-           "final_mse_mean_plain": float(np.mean(loss_p))}
+    bpc = grad_step_bytes_per_cell(C, K, T, ckpt)
+    gbps = bpc * n * n * rate_b / 1e9
+    out = {"config": f"{count} creatures {n}x{n} C{C}K{K} R~U{{10..18}}, T={T}, ckpt {ckpt}, Adam on mu/sigma/w "
+                     f"(device-side), {iters} iterations (plain loop: {pi}, PyTorch reference path: {torch_iters})",
+           "grad_steps_per_s_batched": round(rate_b, 1),
+           "grad_steps_per_s_plain": round(rate_p, 1) if rate_p else None,
+           "ratio": round(rate_b / rate_p, 2) if rate_p else None, "host_round_trips_per_iteration": 0,
+           "final_mse_mean_batched": float(np.mean(loss_b)), "final_mse_mean_plain": float(np.mean(loss_p)),
+           "alg_bytes_per_grad_step": round(bpc * n * n), "alg_GBps": round(gbps, 1),
+           "hbm_frac": round(gbps / HBM_PEAK_GBS, 4), "plan": plan}
     if torch_iters:
         rate_t, loss_t = torch_reference_rate(crs, targets, T, torch_iters)
         out.update({"grad_steps_per_s_torch_reference": round(rate_t, 1), "ratio_vs_torch_reference":
@@ -214,7 +258,7 @@ def batched_grad(pkg, count=32, n=512, C=5, T=32, ckpt=4, iters=50, plain_iters=
     return out
 
 
-def batched_forward(pkg, steps=200, seed=0):
+def batched_forward(pkg, steps=200, seed=0, plans=None):
     """256 creatures of uneven sizes (85 at 256^2, 85 at 512^2, 86 at 1024^2), C ~ U{2,3,4},
     K = C^2, radii U{8..20}, one engine and one graph per step for all of them.  The creatures
     are generated on the device (kernel spectra, growth parameters, seed); the host generator's
@@ -233,7 +277,7 @@ def batched_forward(pkg, steps=200, seed=0):
             pkg.generate_creature(sp["n"], sp["num_channels"], sp["num_kernels"], sp["src"], sp["dst"],
                                   sp["radius"], patch=sp["patch"], seed=sp["seed"], creature=sp["creature"])
         t_host += (time.perf_counter() - t1) * (85 if first < 170 else 86) / 4
-    plans = b.autotune(3)
+    plans = _apply_plans(b, plans) if plans else b.autotune(3)
     b.run_steps(5)
     b.sync()
     t0 = time.perf_counter()
@@ -251,7 +295,7 @@ def batched_forward(pkg, steps=200, seed=0):
             "setup_s_device_generator": round(setup_dev, 3), "setup_s_host_generator_est": round(t_host, 1)}
 
 
-def rollout_grad(pkg, n=1024, R=13, T=64, ckpt=8, iters=5, torch_iters=1, seed=0):
+def rollout_grad(pkg, n=1024, R=13, T=64, ckpt=8, iters=5, torch_iters=1, seed=0, plans=None):
     """Differentiable rollout (config #4): one 1024^2 C3K9 creature, T steps with a checkpoint
     every `ckpt`, MSE to the state a second seeded creature reaches after T steps, gradients for
     the state and mu/sigma/w through the hand-written adjoint (one HIP graph per rollout).
@@ -264,7 +308,7 @@ def rollout_grad(pkg, n=1024, R=13, T=64, ckpt=8, iters=5, torch_iters=1, seed=0
     target = te.download_all_channels().reshape(3, n, n)
     te.close()
     eng = pkg.make_engine(cr)
-    eng.autotune(5)
+    plan = _apply_plans(eng, plans)[0] if plans else eng.autotune(5)[0]
     eng.creature_set_target(0, target)
     eng.rollout_grad(T, ckpt)  # captures the graph
     eng.sync()
@@ -275,8 +319,11 @@ def rollout_grad(pkg, n=1024, R=13, T=64, ckpt=8, iters=5, torch_iters=1, seed=0
     dt = (time.perf_counter() - t0) / iters
     loss = eng.creature_loss(0)
     eng.close()
+    gbps = grad_step_bytes_per_cell(3, 9, T, ckpt) * n * n * T / dt / 1e9
     out = {"config": f"{n}x{n} C3K9 R={R}, T={T}, checkpoint every {ckpt}, {iters} rollouts",
-           "ms_per_rollout": round(dt * 1e3, 3), "grad_steps_per_s": round(T / dt, 1), "loss": loss}
+           "ms_per_rollout": round(dt * 1e3, 3), "grad_steps_per_s": round(T / dt, 1), "loss": loss,
+           "alg_bytes_per_grad_step": round(grad_step_bytes_per_cell(3, 9, T, ckpt) * n * n),
+           "alg_GBps": round(gbps, 1), "hbm_frac": round(gbps / HBM_PEAK_GBS, 4), "plan": plan}
     if torch_iters:
         rate_t, loss_t = torch_reference_rate([cr], [target], T, torch_iters)
         out.update({"grad_steps_per_s_torch_reference": round(rate_t, 1),
@@ -302,16 +349,22 @@ def main():
     ap.add_argument("--radius", type=int, default=26)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-grad", action="store_true",
+                    help="step2048: leave out the batched gradient-search measurement")
     ap.add_argument("--workload", default="step2048",
                     choices=["step2048", "batched-forward", "batched-grad", "rollout"],
                     help="step2048: the headline 2048^2 C3K9 step; the batched workloads print their own line")
     ap.add_argument("--iters", type=int, default=50, help="batched-grad: Adam iterations")
-    ap.add_argument("--plain-iters", type=int, default=0, help="batched-grad: iterations of the plain loop")
+    ap.add_argument("--plain-iters", type=int, default=None,
+                    help="batched-grad: iterations of the plain per-creature loop (default: --iters; 0: skip)")
     ap.add_argument("--torch-iters", type=int, default=2,
                     help="batched-grad: iterations of the PyTorch reference path (0: skip)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--plan", default="",
                     help="fixed launch plan fwd_e,inv_e,col_e,col_tc,diff_th[,col_split] (no autotune)")
+    ap.add_argument("--plans", default="",
+                    help="batched workloads: JSON list of launch plans (as printed under \"plans\") instead "
+                         "of tuning")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a device-to-device copy of this many MiB (PMC calibration only)")
     args = ap.parse_args()
@@ -321,17 +374,7 @@ def main():
         sys.exit(2)
 
     pkg = ge.load_package()
-    if args.workload == "batched-grad":
-        print(json.dumps({"workload": "batched-grad", **batched_grad(pkg, iters=args.iters, torch_iters=args.torch_iters,
-                                                                      plain_iters=args.plain_iters or None)}),
-              flush=True)
-        return
-    if args.workload == "rollout":
-        print(json.dumps({"workload": "rollout", **rollout_grad(pkg, torch_iters=args.torch_iters)}), flush=True)
-        return
-    if args.workload == "batched-forward":
-        print(json.dumps({"workload": "batched-forward", **batched_forward(pkg, steps=args.steps)}), flush=True)
-        return
+    plans = json.loads(args.plans) if args.plans else None
     if args.copy_calibration_mb:
         import torch
         a = torch.ones(args.copy_calibration_mb << 18, dtype=torch.float32, device="cuda:0")
@@ -339,6 +382,20 @@ def main():
         for _ in range(3):
             b.copy_(a)
         torch.cuda.synchronize()
+        del a, b
+    if args.workload == "batched-grad":
+        print(json.dumps({"workload": "batched-grad", **batched_grad(pkg, iters=args.iters, torch_iters=args.torch_iters,
+                                                                      plain_iters=args.plain_iters,
+                                                                      plans=plans)}), flush=True)
+        return
+    if args.workload == "rollout":
+        print(json.dumps({"workload": "rollout", **rollout_grad(pkg, torch_iters=args.torch_iters, plans=plans)}),
+              flush=True)
+        return
+    if args.workload == "batched-forward":
+        print(json.dumps({"workload": "batched-forward", **batched_forward(pkg, steps=args.steps, plans=plans)}),
+              flush=True)
+        return
     n, C, R = args.n, 3, args.radius
     cr = pkg.baseline_creature(n, num_channels=C, radius=R, seed=0)
     K = cr["num_kernels"]
@@ -403,6 +460,11 @@ def main():
                                             "design moves fewer bytes, so above 1 means the model is beaten"},
         "mass_drift_rel_fp64": drift,
     }
+    if not args.no_grad:
+        # The second half of the headline metric: batched forward + adjoint creature search
+        # against the plain per-creature loop (one engine per creature) and the PyTorch path.
+        out["batched_grad"] = batched_grad(pkg, iters=args.iters, torch_iters=args.torch_iters,
+                                           plain_iters=args.plain_iters)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -15,14 +15,18 @@ import os
 import sys
 
 SHORT = {"k_row_fwd": "row_fwd_r2c", "k_col_fused": "col_fused", "k_row_inv_growth": "row_inv_growth",
-         "k_diffusion": "diffusion"}
+         "k_diffusion": "diffusion", "k_diffusion_adj": "diffusion_adj", "k_growth_adj_row": "growth_adj_row",
+         "k_col_adj": "col_adj", "k_row_inv_add": "row_inv_add", "k_param_reduce": "param_reduce",
+         "k_mse_grad": "mse_grad", "k_zero_gparam": "zero_gparam", "k_adam": "adam"}
 
 
 def short(name):
-    for k, v in SHORT.items():
-        if f"cn::{k}" in name:
-            return v
-    return None
+    """Short name of one of this library's kernels (exact base name, template arguments dropped)."""
+    i = name.find("cn::")
+    if i < 0:
+        return None
+    base = name[i + 4:].split("<")[0].split("(")[0]
+    return SHORT.get(base)
 
 
 def variant(name):
