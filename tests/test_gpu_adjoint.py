@@ -62,9 +62,10 @@ def test_rollout_grad_matches_torch_fp64(cn, n, C, steps, ckpt):
             "state0": _rel(g["state0"].reshape(gs_t.shape), gs_t),
             "mu": _rel(g["mu"], gm_t), "sigma": _rel(g["sigma"], gsg_t), "weights": _rel(g["weights"], gw_t)}
     print(f"n={n} C={C} T={steps} ckpt={ckpt}: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
-    assert errs["loss"] <= 1e-5
+    # bars about 10x the measured errors (loss <= 1.7e-8, gradients <= 8.1e-7)
+    assert errs["loss"] <= 2e-7
     for k in ("state0", "mu", "sigma", "weights"):
-        assert errs[k] <= 1e-4, (k, errs[k])
+        assert errs[k] <= 1e-5, (k, errs[k])
 
 
 def test_rollout_grad_with_shifted_kernels(cn):
@@ -80,9 +81,10 @@ def test_rollout_grad_with_shifted_kernels(cn):
             "state0": _rel(g["state0"].reshape(gs_t.shape), gs_t),
             "mu": _rel(g["mu"], gm_t), "sigma": _rel(g["sigma"], gsg_t), "weights": _rel(g["weights"], gw_t)}
     print("shifted kernels: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
-    assert errs["loss"] <= 1e-5
+    # measured: loss 2.5e-9, gradients <= 1.7e-6
+    assert errs["loss"] <= 1e-7
     for k in ("state0", "mu", "sigma", "weights"):
-        assert errs[k] <= 1e-4, (k, errs[k])
+        assert errs[k] <= 2e-5, (k, errs[k])
 
 
 @pytest.mark.parametrize("maps", ["c1k1", "ragged_c3k5", "c4k2"])
@@ -102,9 +104,10 @@ def test_rollout_grad_ragged_maps(cn, maps):
             "state0": _rel(g["state0"].reshape(gs_t.shape), gs_t),
             "mu": _rel(g["mu"], gm_t), "sigma": _rel(g["sigma"], gsg_t), "weights": _rel(g["weights"], gw_t)}
     print(f"{maps}: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
-    assert errs["loss"] <= 1e-5
+    # measured: loss <= 8e-9, gradients <= 3.5e-7
+    assert errs["loss"] <= 1e-7
     for k in ("state0", "mu", "sigma", "weights"):
-        assert errs[k] <= 1e-4, (k, errs[k])
+        assert errs[k] <= 5e-6, (k, errs[k])
 
 
 def test_checkpoint_stride_does_not_change_gradients(cn):
@@ -169,8 +172,8 @@ def test_param_grads_vs_fp64_finite_differences(cn):
         args_p[j][idx] += h
         args_m[j][idx] -= h
         fd = (loss_of(*args_p) - loss_of(*args_m)) / (2 * h)
-        print(f"d loss / d {name}[{idx}]: HIP {g[name][idx]:.6e}  FD(fp64) {fd:.6e}")
-        assert abs(g[name][idx] - fd) <= 1e-3 * abs(fd) + 1e-9
+        print(f"d loss / d {name}[{idx}]: HIP {g[name][idx]:.6e}  FD(fp64) {fd:.6e}  rel {abs(g[name][idx] - fd) / abs(fd):.2e}")
+        assert abs(g[name][idx] - fd) <= 1e-5 * abs(fd)  # agree to 7 significant digits
 
 
 def test_split_backward_unaffected_by_conv_readback(cn):
@@ -221,13 +224,24 @@ def test_autograd_function_matches_internal_mse(cn):
     assert _rel(s0.grad.cpu().numpy().reshape(-1), g["state0"].astype(np.float64)) <= 1e-5
     for name, t in (("mu", mu), ("sigma", sg), ("weights", w)):
         assert _rel(t.grad.cpu().numpy(), g[name].astype(np.float64)) <= 1e-5
+    # and directly against torch autograd of the float64 restatement
+    loss_t, gs_t, gm_t, gsg_t, gw_t = _torch_grads(cr, tg["state"], 5)
+    errs = {"loss": abs(float(loss) - loss_t) / loss_t, "state0": _rel(s0.grad.cpu().numpy(), gs_t),
+            "mu": _rel(mu.grad.cpu().numpy(), gm_t), "sigma": _rel(sg.grad.cpu().numpy(), gsg_t),
+            "weights": _rel(w.grad.cpu().numpy(), gw_t)}
+    print("autograd.rollout vs torch FP64 autograd: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+    # measured: loss 1.6e-8, gradients <= 2.2e-7
+    assert errs["loss"] <= 2e-7
+    for k in ("state0", "mu", "sigma", "weights"):
+        assert errs[k] <= 5e-6, (k, errs[k])
     eng.close()
 
 
 def test_differentiable_rollout_1024_c3k9_t64(cn):
     """Differentiable rollout at 1024^2, C3K9, T=64, checkpoint every 8: MSE to a second seeded
     creature's state after 64 steps; gradients for the 3x128x128 seed patch and mu/sigma/w
-    against float64 torch autograd of the reference restatement."""
+    against float64 torch autograd of the reference restatement, and against FP64 central finite
+    differences (h = 1e-4) of the same restatement for all 27 growth parameters and 8 seed pixels."""
     n, T = 1024, 64
     cr, tg = _pair(cn, n, 3, R=13, seed=0, patch=128)
     eng = cn.make_engine(tg)
@@ -241,10 +255,52 @@ def test_differentiable_rollout_1024_c3k9_t64(cn):
     patch = g["state0"].reshape(3, n, n)[:, sl, sl]
     errs = {"loss": abs(loss - loss_t) / loss_t, "seed_patch": _rel(patch, patch_t), "mu": _rel(g["mu"], gm_t),
             "sigma": _rel(g["sigma"], gsg_t), "weights": _rel(g["weights"], gw_t)}
-    print("1024^2 C3K9 T=64 ckpt 8: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
-    assert errs["loss"] <= 1e-4
+    print("1024^2 C3K9 T=64 ckpt 8, engine vs torch FP64 autograd: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+
+    # FP64 central finite differences (h = 1e-4) of the float64 restatement at the same shape:
+    # every mu, sigma and weight, and the 8 seed-patch pixels with the largest gradient.
+    kf, src, dst, mu, sg, w = torch_ref.creature_tensors(cr, DEV)
+    s0 = torch.as_tensor(cr["state"].astype(np.float64), device=DEV)
+    tgt = torch.as_tensor(target.astype(np.float64), device=DEV)
+
+    def loss_of(s, m, sgm, ww):
+        with torch.no_grad():
+            return float(((torch_ref.rollout(s, kf, src, dst, m, sgm, ww, T) - tgt) ** 2).mean())
+
+    h = 1e-4
+    fd = {}
+    for j, name in enumerate(("mu", "sigma", "weights")):
+        vals = []
+        for k in range(9):
+            args_p, args_m = [mu.clone(), sg.clone(), w.clone()], [mu.clone(), sg.clone(), w.clone()]
+            args_p[j][k] += h
+            args_m[j][k] -= h
+            vals.append((loss_of(s0, *args_p) - loss_of(s0, *args_m)) / (2 * h))
+        fd[name] = np.array(vals)
+    flat = np.argsort(-np.abs(patch.reshape(-1)))[:8]
+    pix = [np.unravel_index(f, patch.shape) for f in flat]
+    vals = []
+    for (c, y, x) in pix:
+        sp, sm = s0.clone(), s0.clone()
+        sp[c, n // 2 - 64 + y, n // 2 - 64 + x] += h
+        sm[c, n // 2 - 64 + y, n // 2 - 64 + x] -= h
+        vals.append((loss_of(sp, mu, sg, w) - loss_of(sm, mu, sg, w)) / (2 * h))
+    fd["pixels"] = np.array(vals)
+    eng_v = {"mu": g["mu"], "sigma": g["sigma"], "weights": g["weights"], "pixels": patch.reshape(-1)[flat]}
+    tor_v = {"mu": gm_t, "sigma": gsg_t, "weights": gw_t, "pixels": patch_t.reshape(-1)[flat]}
+    fd_e = {k: _rel(eng_v[k], fd[k]) for k in fd}
+    fd_t = {k: _rel(tor_v[k], fd[k]) for k in fd}
+    print("engine vs FP64 central differences: " + ", ".join(f"{k} {v:.2e}" for k, v in fd_e.items()))
+    print("torch FP64 autograd vs FP64 central differences: " + ", ".join(f"{k} {v:.2e}" for k, v in fd_t.items()))
+    # measured: engine vs autograd loss 3.8e-8, gradients <= 2.2e-6; engine vs differences
+    # <= 3.7e-6; autograd vs differences 3.9e-6 for mu and sigma (the differences' own
+    # truncation error), <= 2.5e-8 for the weights and pixels
+    assert errs["loss"] <= 1e-6
     for k in ("seed_patch", "mu", "sigma", "weights"):
-        assert errs[k] <= 1e-3, (k, errs[k])
+        assert errs[k] <= 1e-5, (k, errs[k])
+    for k in fd:
+        assert fd_t[k] <= 2e-5, ("torch", k, fd_t[k])
+        assert fd_e[k] <= 2e-5, ("engine", k, fd_e[k])
 
 
 def test_rollout_grad_maximum_channels_and_kernels(cn):
@@ -256,9 +312,10 @@ def test_rollout_grad_maximum_channels_and_kernels(cn):
             "state0": _rel(g["state0"].reshape(gs_t.shape), gs_t),
             "mu": _rel(g["mu"], gm_t), "sigma": _rel(g["sigma"], gsg_t), "weights": _rel(g["weights"], gw_t)}
     print("32^2 C16K256 T=3 ckpt=2: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
-    assert errs["loss"] <= 1e-5
+    # measured: loss 1.2e-8, gradients <= 3e-7
+    assert errs["loss"] <= 2e-7
     for k in ("state0", "mu", "sigma", "weights"):
-        assert errs[k] <= 1e-4, (k, errs[k])
+        assert errs[k] <= 5e-6, (k, errs[k])
 
 
 def test_autograd_function_cpu_parameters_and_superseded_forward(cn):
@@ -300,6 +357,8 @@ def test_rollout_grad_mixed_row_geometries(cn, n):
         errs = {"state0": _rel(g["state0"].reshape(gs_t.shape), gs_t), "mu": _rel(g["mu"], gm_t),
                 "sigma": _rel(g["sigma"], gsg_t), "weights": _rel(g["weights"], gw_t)}
         print(f"n={n} fwd_e={fe} inv_e={ie}: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+        # measured: <= 4.5e-7 at 256^2; at 1024^2 the FP32 parameter sums over 2^20 sites per
+        # step reach 9.5e-6
         for k, v in errs.items():
-            assert v <= 1e-4, (fe, ie, k, v)
+            assert v <= (5e-6 if n <= 256 else 1e-4), (fe, ie, k, v)
     eng.close()
