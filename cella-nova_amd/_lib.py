@@ -100,6 +100,8 @@ SIGNATURES = {
     "cn_write_kernels": (c_int, [c_char_p, c_void_p, c_uint32, c_uint32]),
     "cn_write_config": (c_int, [c_char_p, POINTER(Config)]),
     "cn_creature_generate": (c_int, [c_void_p, c_uint32, POINTER(CreatureParams)]),
+    "cn_generate_example_creature": (c_int, [c_uint32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                             c_void_p, c_void_p]),
     "cn_creature_download_kernels": (c_int, [c_void_p, c_uint32, c_void_p, c_size_t]),
     "cn_generate_creature": (c_int, [POINTER(CreatureParams), c_void_p, c_void_p, c_void_p, c_void_p,
                                      c_void_p, c_void_p]),

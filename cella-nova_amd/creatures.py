@@ -112,3 +112,29 @@ def search_batch_creatures(count=32, n=512, num_channels=5, seed=0):
     tgs = [baseline_creature(n, num_channels=num_channels, radius=int(radii[i]), seed=seed, creature=1000 + i)
            for i in range(count)]
     return crs, tgs
+
+
+def example_creature(n=1024, spectra=True, spatial=False, state=True):
+    """The creature of the reference's CPU example, ml-rs-example (ml-rs/example.rs:310-491):
+    C = 3, K = 9, c0 = k % 3 (input), c1 = k // 3 (output), sigmoid x Gaussian ring kernels,
+    Gaussian-blob seed, the example's growth parameters; at grid size n (the example uses 1024)."""
+    K, C, P = 9, 3, n * n
+    kf = np.empty(2 * K * P, np.float32) if spectra else None
+    sp = np.empty(K * P, np.float64) if spatial else None
+    st = np.empty(C * P, np.float32) if state else None
+    mu, sg, w = (np.empty(K, np.float32) for _ in range(3))
+    src, dst = np.empty(K, np.uint32), np.empty(K, np.uint32)
+
+    def ptr(a):
+        return a.ctypes.data if a is not None else None
+
+    check(lib().cn_generate_example_creature(n, ptr(kf), ptr(sp), ptr(st), mu.ctypes.data, sg.ctypes.data,
+                                             w.ctypes.data, src.ctypes.data, dst.ctypes.data))
+    out = {"n": n, "num_channels": C, "num_kernels": K, "src": src, "dst": dst, "mu": mu, "sigma": sg, "weights": w}
+    if spectra:
+        out["kernels_fft"] = kf.view(np.complex64).reshape(K, P)
+    if spatial:
+        out["spatial"] = sp.reshape(K, n, n)
+    if state:
+        out["state"] = st.reshape(C, n, n)
+    return out

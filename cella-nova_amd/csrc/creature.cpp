@@ -154,3 +154,90 @@ extern "C" cn_status cn_generate_creature(const cn_creature_params* p, float* ke
   }
   return CN_OK;
 }
+
+// ---------------------------------------------------------------------------------------
+// The creature of the reference's CPU example (ml-rs/example.rs:310-491), at any grid size n.
+// ---------------------------------------------------------------------------------------
+namespace {
+
+constexpr int kExC = 3, kExK = 9;
+// Ring radii and widths of the example's 9 kernels, as fractions of global_r = 10 (:347-353).
+constexpr double kExRadius[kExK] = {0.25, 0.3167, 0.3833, 0.45, 0.5167, 0.5833, 0.65, 0.7167, 0.85};
+constexpr double kExWidth[kExK] = {0.03, 0.0411, 0.0522, 0.0633, 0.0744, 0.0856, 0.0967, 0.1078, 0.13};
+// Seed blobs (sigma, x offset, y offset) per channel (:430).
+constexpr double kExBlob[kExC][3] = {{0.25, 0.0, 0.0}, {0.22, 0.04, 0.0}, {0.28, 0.0, 0.04}};
+
+// sigmoid(x) = tanh(x / 2) / 2 + 1/2 (:310-312).
+double ex_sigmoid(double x) { return 0.5 * std::tanh(x * 0.5) + 0.5; }
+
+// Kernel k at the grid point (i, j) of the centred layout: a Gaussian ring at half the scaled
+// radius under a sigmoid envelope that ends at the scaled radius (:362-383).
+double ex_kernel_value(int k, size_t i, size_t j, size_t n) {
+  const double mid = (double)(n / 2), di = (double)i - mid, dj = (double)j - mid;
+  const double ds = std::sqrt(di * di + dj * dj) / (10.0 * kExRadius[k]);
+  const double diff = ds - 0.5, w = kExWidth[k];
+  return ex_sigmoid(-(ds - 1.0) * 10.0) * std::exp(-(diff * diff) / (2.0 * w * w));
+}
+
+}  // namespace
+
+// Kernels (:338-419): each ring kernel normalised to sum 1 (:390-397), moved so that its centre
+// (n/2, n/2) sits at the origin (ifftshift, :136-152, :408: output (y, x) takes the centred value at
+// ((y + n/2) mod n, (x + n/2) mod n)), and transformed (:414; FP64 here, rounded to complex64).
+// Seed (:427-458): channel c = exp(-((x - ox)^2 + (y - oy)^2) / (2 s^2)) with x = (j - n/2)/(n/2),
+// y = (i - n/2)/(n/2).  Growth (:466-491) in the example's f32 arithmetic: kernel k reads channel
+// k % 3 and writes channel k / 3, perm = 3 (k % 3) + k / 3, mu = 0.05 + 0.20 perm / 9,
+// sigma = 0.03 + 0.12 perm / 9, weight 0.5 when input and output agree, else 0.25.
+extern "C" cn_status cn_generate_example_creature(uint32_t n, float* kernels_fft, double* spatial, float* state,
+                                                  float* mu, float* sigma, float* weights, uint32_t* src,
+                                                  uint32_t* dst) {
+  clear_error();
+  if (!is_pow2(n) || n < 4 || n > 8192) return set_error(CN_ERR_UNSUPPORTED, "n = %u must be a power of two in [4, 8192]", n);
+  const size_t N = n, P = N * N, h = N / 2;
+  for (int k = 0; k < kExK; ++k) {
+    const int in = k % kExC, out = k / kExC;
+    const float perm = (float)(in * kExC + out);
+    if (src) src[k] = (uint32_t)in;
+    if (dst) dst[k] = (uint32_t)out;
+    if (mu) mu[k] = 0.05f + 0.20f * perm / (float)kExK;
+    if (sigma) sigma[k] = 0.03f + 0.12f * perm / (float)kExK;
+    if (weights) weights[k] = in == out ? 0.5f : 0.25f;
+  }
+  if (state) {
+    const double half = (double)N / 2.0;
+    for (int c = 0; c < kExC; ++c) {
+      const double s = kExBlob[c][0], ox = kExBlob[c][1], oy = kExBlob[c][2], inv = 1.0 / (2.0 * s * s);
+      for (size_t i = 0; i < N; ++i)
+        for (size_t j = 0; j < N; ++j) {
+          const double dx = ((double)j - half) / half - ox, dy = ((double)i - half) / half - oy;
+          state[c * P + i * N + j] = (float)std::exp(-(dx * dx + dy * dy) * inv);
+        }
+    }
+  }
+  if (!kernels_fft && !spatial) return CN_OK;
+  auto build = [&](int k) {
+    std::vector<double> re(P), im(P, 0.0);
+    double tot = 0.0;
+    for (size_t y = 0; y < N; ++y)
+      for (size_t x = 0; x < N; ++x) {
+        const double v = ex_kernel_value(k, (y + h) & (N - 1), (x + h) & (N - 1), N);
+        re[y * N + x] = v;
+        tot += v;
+      }
+    if (tot > 0.0)
+      for (size_t i = 0; i < P; ++i) re[i] /= tot;
+    if (spatial) std::memcpy(spatial + (size_t)k * P, re.data(), P * sizeof(double));
+    if (kernels_fft) {
+      fft2d(re, im, N);
+      float* o = kernels_fft + (size_t)k * P * 2;
+      for (size_t i = 0; i < P; ++i) {
+        o[2 * i] = (float)re[i];
+        o[2 * i + 1] = (float)im[i];
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (int k = 0; k < kExK; ++k) th.emplace_back(build, k);
+  for (auto& t : th) t.join();
+  return CN_OK;
+}

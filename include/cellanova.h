@@ -322,6 +322,19 @@ typedef struct cn_creature_params {
 cn_status cn_generate_creature(const cn_creature_params* p, float* kernels_fft, double* spatial,
                                float* state, float* mu, float* sigma, float* weights);
 
+/* The creature of the reference's CPU example (ml-rs/example.rs:310-491: generate_kernels_fft,
+ * ifftshift_2d, generate_seed, generate_growth_params), at grid size n (the example fixes
+ * n = 1024; C = 3 and K = 9 always).  Kernel k: a Gaussian ring of width w_k at half the scaled
+ * radius d / (10 r_k), under a sigmoid envelope ending at the scaled radius 1, with the example's
+ * r and w tables; normalised to sum 1 and moved so its centre sits at the origin.  Outputs (any
+ * may be NULL): kernels_fft [9][n*n] interleaved complex64 (FP64 transform, rounded), spatial
+ * [9][n*n] f64 (the origin-centred kernels whose transform that is), state [3][n*n] f32 (three
+ * Gaussian blobs), mu/sigma/weights [9] (the example's f32 values), src/dst [9] (k % 3, k / 3:
+ * the example's c0 = input, c1 = output).  Kernels and seed are evaluated in FP64 (the example
+ * uses f32; the two agree to f32 rounding). */
+cn_status cn_generate_example_creature(uint32_t n, float* kernels_fft, double* spatial, float* state, float* mu,
+                                       float* sigma, float* weights, uint32_t* src, uint32_t* dst);
+
 /* New (the reference generates spectra on the host, train/generate_kernel_json.py:106-145,
  * and uploads them with set_kernel, mod.rs:207): generate creature i of an engine in place on
  * the device from the distribution above.  Kernels are evaluated and normalised in FP64 on the

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -90,11 +91,14 @@ void fft_2d(Cpx<T>* g, size_t n) {
   }
 }
 
-// example.rs:179-199 - convolution of every kernel (source FFT redone per kernel).
+// example.rs:179-199 - convolution of every kernel (source FFT redone per kernel).  With
+// threads > 1 the kernels are shared out over threads; each writes only its own buffer, so the
+// results are the same bits as with one thread (the reference runs one).
 template <typename T>
-void conv_all(size_t n, size_t K, const T* channels, const T* kfft, const uint32_t* src, Cpx<T>* conv) {
+void conv_all(size_t n, size_t K, const T* channels, const T* kfft, const uint32_t* src, Cpx<T>* conv,
+              int threads = 1) {
   const size_t P = n * n;
-  for (size_t k = 0; k < K; ++k) {
+  auto one = [&](size_t k) {
     Cpx<T>* g = conv + k * P;
     const T* ch = channels + src[k] * P;
     for (size_t i = 0; i < P; ++i) g[i] = {ch[i], (T)0};
@@ -102,7 +106,17 @@ void conv_all(size_t n, size_t K, const T* channels, const T* kfft, const uint32
     const T* kf = kfft + k * P * 2;
     for (size_t i = 0; i < P; ++i) g[i] = cmul(g[i], Cpx<T>{kf[2 * i], kf[2 * i + 1]});
     fft_2d<T, true>(g, n);
+  };
+  if (threads <= 1) {
+    for (size_t k = 0; k < K; ++k) one(k);
+    return;
   }
+  std::vector<std::thread> th;
+  for (int t = 0; t < threads; ++t)
+    th.emplace_back([&, t] {
+      for (size_t k = (size_t)t; k < K; k += (size_t)threads) one(k);
+    });
+  for (auto& x : th) x.join();
 }
 
 // example.rs:228-266 - bump growth, weighted sum per output channel, A = exp(temp * wsum).
@@ -151,10 +165,10 @@ void diffusion_phase(size_t n, const T* affinity, const T* channel, T* out) {
 // example.rs:164-222 - one synchronous step.
 template <typename T>
 void step_ml(size_t n, size_t C, size_t K, T* channels, const T* kfft, const uint32_t* src, const uint32_t* dst,
-             const T* mu, const T* sigma, const T* w, T temp) {
+             const T* mu, const T* sigma, const T* w, T temp, int threads = 1) {
   const size_t P = n * n;
   std::vector<Cpx<T>> conv(K * P);
-  conv_all(n, K, channels, kfft, src, conv.data());
+  conv_all(n, K, channels, kfft, src, conv.data(), threads);
   std::vector<T> aff(P), tmp(P);
   for (size_t c = 0; c < C; ++c) {
     growth_phase(n, K, conv.data(), dst, mu, sigma, w, temp, (uint32_t)c, aff.data());
@@ -178,6 +192,14 @@ extern "C" {
 void oracle_run_f32(int n, int C, int K, float* channels, const float* kfft, const uint32_t* src,
                     const uint32_t* dst, const float* mu, const float* sigma, const float* w, float temp, int steps) {
   for (int s = 0; s < steps; ++s) step_ml<float>(n, C, K, channels, kfft, src, dst, mu, sigma, w, temp);
+}
+
+// oracle_run_f32 with the kernels' convolutions shared out over `threads` threads (the same
+// results; for tests that run many full-size steps).
+void oracle_run_f32_threads(int n, int C, int K, float* channels, const float* kfft, const uint32_t* src,
+                            const uint32_t* dst, const float* mu, const float* sigma, const float* w, float temp,
+                            int steps, int threads) {
+  for (int s = 0; s < steps; ++s) step_ml<float>(n, C, K, channels, kfft, src, dst, mu, sigma, w, temp, threads);
 }
 
 // Same in FP64 (kernel spectra and parameters widened from f32).
@@ -221,6 +243,69 @@ void oracle_growth_f64(int n, int K, const double* u, const uint32_t* dst, const
 // Diffusion of one channel given its affinity (FP64).
 void oracle_diffusion_f64(int n, const double* affinity, const double* channel, double* out) {
   diffusion_phase<double>(n, affinity, channel, out);
+}
+
+// The reference CPU example's creature in its own f32 arithmetic (example.rs:310-491, with
+// N = n): sigmoid (:310-312), ring kernels (:338-419: f32 distance from the grid centre,
+// sigmoid envelope times Gaussian ring, f32 sum in row-major order, division, ifftshift by a
+// quadrant swap :136-152, the f32 radix-2 fft_2d above), seed (:427-458) and growth parameters
+// (:466-491).  kernels_fft [9][n][n] interleaved, seed [3][n][n], mu/sigma/w [9], c0/c1 [9].
+void oracle_example_creature_f32(int n, float* kernels_fft, float* seed, float* mu, float* sigma, float* w,
+                                 uint32_t* c0, uint32_t* c1) {
+  const size_t N = (size_t)n, P = N * N, h = N / 2;
+  const int C = 3, K = 9;
+  const float radii[K] = {0.25f, 0.3167f, 0.3833f, 0.45f, 0.5167f, 0.5833f, 0.65f, 0.7167f, 0.85f};
+  const float widths[K] = {0.03f, 0.0411f, 0.0522f, 0.0633f, 0.0744f, 0.0856f, 0.0967f, 0.1078f, 0.13f};
+  auto sigmoid = [](float x) { return 0.5f * std::tanh(x * 0.5f) + 0.5f; };
+  const float mid = (float)(N / 2), global_r = 10.0f;
+  std::vector<float> spatial(P);
+  std::vector<Cpx<float>> grid(P);
+  for (int k = 0; k < K; ++k) {
+    for (size_t i = 0; i < N; ++i)
+      for (size_t j = 0; j < N; ++j) {
+        const float di = (float)i - mid, dj = (float)j - mid;
+        const float dist = std::sqrt(di * di + dj * dj);
+        const float d_scaled = dist / (global_r * radii[k]);
+        const float sig = sigmoid(-(d_scaled - 1.0f) * 10.0f);
+        const float diff = d_scaled - 0.5f;
+        const float ker_val = std::exp(-(diff * diff) / (2.0f * widths[k] * widths[k]));
+        spatial[i * N + j] = sig * ker_val;
+      }
+    float total = 0.0f;
+    for (size_t i = 0; i < P; ++i) total += spatial[i];
+    if (total > 0.0f)
+      for (size_t i = 0; i < P; ++i) spatial[i] /= total;
+    for (size_t i = 0; i < P; ++i) grid[i] = {spatial[i], 0.0f};
+    for (size_t i = 0; i < h; ++i)  // ifftshift_2d: swap quadrants 0 <-> 3, then 1 <-> 2
+      for (size_t j = 0; j < h; ++j) std::swap(grid[i * N + j], grid[(i + h) * N + j + h]);
+    for (size_t i = 0; i < h; ++i)
+      for (size_t j = 0; j < h; ++j) std::swap(grid[i * N + j + h], grid[(i + h) * N + j]);
+    fft_2d<float, false>(grid.data(), N);
+    for (size_t i = 0; i < P; ++i) {
+      kernels_fft[((size_t)k * P + i) * 2] = grid[i].re;
+      kernels_fft[((size_t)k * P + i) * 2 + 1] = grid[i].im;
+    }
+  }
+  const float blobs[C][3] = {{0.25f, 0.0f, 0.0f}, {0.22f, 0.04f, 0.0f}, {0.28f, 0.0f, 0.04f}};
+  const float half = (float)N / 2.0f;
+  for (int c = 0; c < C; ++c) {
+    const float inv_sigma2 = 1.0f / (2.0f * blobs[c][0] * blobs[c][0]);
+    for (size_t i = 0; i < N; ++i)
+      for (size_t j = 0; j < N; ++j) {
+        const float x = ((float)j - half) / half, y = ((float)i - half) / half;
+        const float dx = x - blobs[c][1], dy = y - blobs[c][2];
+        seed[c * P + i * N + j] = std::exp(-(dx * dx + dy * dy) * inv_sigma2);
+      }
+  }
+  for (int k = 0; k < K; ++k) {
+    const int in = k % C, out = k / C;
+    c0[k] = (uint32_t)in;
+    c1[k] = (uint32_t)out;
+    const float perm = (float)(in * C + out);
+    mu[k] = 0.05f + 0.20f * perm / (float)K;
+    sigma[k] = 0.03f + 0.12f * perm / (float)K;
+    w[k] = in == out ? 0.5f : 0.25f;
+  }
 }
 
 // 2D forward FFT (FP64 or the reference's f32 arithmetic), interleaved in place.

@@ -31,6 +31,9 @@ _SIGS = {
     "oracle_diffusion_f64": [ctypes.c_int, _P, _P, _P],
     "oracle_fft2_f64": [ctypes.c_int, _P],
     "oracle_fft2_f32": [ctypes.c_int, _P],
+    "oracle_run_f32_threads": [ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P, _P, _P, ctypes.c_float,
+                               ctypes.c_int, ctypes.c_int],
+    "oracle_example_creature_f32": [ctypes.c_int, _P, _P, _P, _P, _P, _P, _P],
 }
 
 
@@ -56,16 +59,36 @@ def _kf(kfft, K, n):
     return a.view(np.float32)
 
 
-def run_f32(state, kfft, src, dst, mu, sigma, w, temp=1.0, steps=1):
-    """`steps` steps of ml-rs-example's f32 arithmetic; state [C,n,n] -> new f32 array."""
+def run_f32(state, kfft, src, dst, mu, sigma, w, temp=1.0, steps=1, threads=1):
+    """`steps` steps of ml-rs-example's f32 arithmetic; state [C,n,n] -> new f32 array.  threads > 1
+    shares the kernels' convolutions out over threads (the same results as one thread)."""
     st = _c(state, np.float32).copy()
     C, n, _ = st.shape
     K = len(src)
     kf, s_, d_ = _kf(kfft, K, n), _c(src, np.uint32), _c(dst, np.uint32)
     m, sg, ww = _c(mu, np.float32), _c(sigma, np.float32), _c(w, np.float32)
-    lib().oracle_run_f32(n, C, K, st.ctypes.data, kf.ctypes.data, s_.ctypes.data, d_.ctypes.data, m.ctypes.data,
-                         sg.ctypes.data, ww.ctypes.data, float(temp), int(steps))
+    if threads > 1:
+        lib().oracle_run_f32_threads(n, C, K, st.ctypes.data, kf.ctypes.data, s_.ctypes.data, d_.ctypes.data,
+                                     m.ctypes.data, sg.ctypes.data, ww.ctypes.data, float(temp), int(steps),
+                                     int(threads))
+    else:
+        lib().oracle_run_f32(n, C, K, st.ctypes.data, kf.ctypes.data, s_.ctypes.data, d_.ctypes.data, m.ctypes.data,
+                             sg.ctypes.data, ww.ctypes.data, float(temp), int(steps))
     return st
+
+
+def example_creature_f32(n=1024):
+    """The reference CPU example's creature in its own f32 arithmetic (example.rs:310-491): dict with
+    kernels_fft (9, n*n) complex64, state (3, n, n) f32, mu/sigma/weights (9,) f32, src/dst (9,)."""
+    K, C, P = 9, 3, n * n
+    kf = np.empty(2 * K * P, np.float32)
+    st = np.empty(C * P, np.float32)
+    mu, sg, w = (np.empty(K, np.float32) for _ in range(3))
+    c0, c1 = np.empty(K, np.uint32), np.empty(K, np.uint32)
+    lib().oracle_example_creature_f32(n, kf.ctypes.data, st.ctypes.data, mu.ctypes.data, sg.ctypes.data,
+                                      w.ctypes.data, c0.ctypes.data, c1.ctypes.data)
+    return {"n": n, "num_channels": C, "num_kernels": K, "kernels_fft": kf.view(np.complex64).reshape(K, P),
+            "state": st.reshape(C, n, n), "mu": mu, "sigma": sg, "weights": w, "src": c0, "dst": c1}
 
 
 def run_f64(state, kfft, src, dst, mu, sigma, w, temp=1.0, steps=1):

@@ -1,4 +1,4 @@
-"""Writes the committed golden fixtures tests/golden/step_*.npz.
+"""Writes the committed golden fixtures tests/golden/step_*.npz and example_n16.npz.
 
 Inputs are small synthetic creatures (seeded generator of the package).  Expected outputs are
 computed by brute force in FP64 with no FFT anywhere: the effective spatial kernel is the real
@@ -35,6 +35,38 @@ def brute_step(state, kfft, src, dst, mu, sigma, w, temp):
     return A * orc.box3(state / Z), u
 
 
+def example_creature_brute(n):
+    """The reference CPU example's creature (ml-rs/example.rs:310-491) restated in FP64 numpy
+    from its formulas, with a brute-force matrix DFT: spatial kernels (centre (n/2, n/2),
+    sigmoid x Gaussian ring, sum 1), the ifftshift as an explicit index map (output (y, x)
+    takes the centred value at ((y + n/2) mod n, (x + n/2) mod n)), seed blobs, growth
+    parameters in float32 arithmetic."""
+    orc = ge.load_oracle()
+    radii = [0.25, 0.3167, 0.3833, 0.45, 0.5167, 0.5833, 0.65, 0.7167, 0.85]
+    widths = [0.03, 0.0411, 0.0522, 0.0633, 0.0744, 0.0856, 0.0967, 0.1078, 0.13]
+    h = n // 2
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    spatial, spectra = [], []
+    for r, w in zip(radii, widths):
+        d = np.sqrt((ii - h) ** 2.0 + (jj - h) ** 2.0) / (10.0 * r)
+        cen = (0.5 * np.tanh(-(d - 1.0) * 10.0 * 0.5) + 0.5) * np.exp(-(d - 0.5) ** 2 / (2.0 * w * w))
+        cen = cen / cen.sum()
+        shifted = cen[(ii + h) % n, (jj + h) % n]
+        spatial.append(shifted)
+        spectra.append(orc.dft2_direct(shifted.astype(np.complex128)))
+    x, y = (jj - h) / h, (ii - h) / h
+    seed = np.stack([np.exp(-((x - ox) ** 2 + (y - oy) ** 2) / (2.0 * s * s))
+                     for s, ox, oy in ((0.25, 0.0, 0.0), (0.22, 0.04, 0.0), (0.28, 0.0, 0.04))])
+    f = np.float32
+    k = np.arange(9)
+    perm = ((k % 3) * 3 + k // 3).astype(f)
+    mu = f(0.05) + f(0.20) * perm / f(9)
+    sigma = f(0.03) + f(0.12) * perm / f(9)
+    weights = np.where(k % 3 == k // 3, f(0.5), f(0.25)).astype(f)
+    return (np.stack(spatial), np.stack(spectra), seed, mu.astype(f), sigma.astype(f), weights,
+            (k % 3).astype(np.uint32), (k // 3).astype(np.uint32))
+
+
 def main():
     pkg = ge.load_package()
     cases = [
@@ -54,6 +86,15 @@ def main():
                             src=cr["src"], dst=cr["dst"], mu=cr["mu"], sigma=cr["sigma"], weights=cr["weights"],
                             temp=np.float32(temp), conv0=u0, state1=s1, state2=s2)
         print(name, "mass", s0.sum(), s1.sum(), s2.sum())
+    # The reference CPU example's own creature at n = 16 and one brute-force step from it.
+    spatial, spectra, seed, mu, sigma, w, src, dst = example_creature_brute(16)
+    state = seed.astype(np.float32)
+    kf = spectra.reshape(9, -1).astype(np.complex64)
+    s1, _ = brute_step(state.astype(np.float64), kf, src, dst, mu, sigma, w, 1.0)
+    np.savez_compressed(os.path.join(HERE, "example_n16.npz"), spatial=spatial, spectra=spectra.reshape(9, -1),
+                        seed=seed, state=state, kernels_fft=kf, mu=mu, sigma=sigma, weights=w, src=src, dst=dst,
+                        state1=s1)
+    print("example_n16", "mass", state.sum(), s1.sum())
 
 
 if __name__ == "__main__":
