@@ -25,6 +25,7 @@
 // copies of the same values laid out for coalesced reads (types.h, tw_pass_offset).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "types.h"
 
 namespace cn {
@@ -402,7 +403,17 @@ __device__ __forceinline__ void fft_run(float2* v, float2* lds, int t, int lane,
 }
 
 // Position (within the transform) of register slot i for the first pass's inputs and the
-// last pass's outputs.
+// last pass's outputs: thread t holds position t + in_off(i) / t + out_off(i).
+template <int L, int E, bool INV>
+__host__ __device__ constexpr int in_off(int i) {
+  using PL = Plan<L, E>;
+  return (i / PL::first_radix(INV)) * PL::T + (i % PL::first_radix(INV)) * (L / PL::first_radix(INV));
+}
+template <int L, int E, bool INV>
+__host__ __device__ constexpr int out_off(int i) {
+  using PL = Plan<L, E>;
+  return (i / PL::last_radix(INV)) * PL::T + (i % PL::last_radix(INV)) * (L / PL::last_radix(INV));
+}
 template <int L, int E, bool INV>
 __device__ __forceinline__ int in_pos(int t, int i) {
   using PL = Plan<L, E>;
@@ -414,6 +425,16 @@ __device__ __forceinline__ int out_pos(int t, int i) {
   using PL = Plan<L, E>;
   constexpr int R = PL::last_radix(INV);
   return t + (i / R) * PL::T + (i % R) * (L / R);
+}
+
+// f(std::integral_constant<int, I>{}) for I = 0 .. n-1: a loop whose index is a constant
+// expression in the body (compile-time choices per register slot).
+template <int I, int n, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (I < n) {
+    f(std::integral_constant<int, I>{});
+    static_for<I + 1, n>(f);
+  }
 }
 
 }  // namespace cn

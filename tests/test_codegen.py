@@ -21,7 +21,13 @@ HOT = ("k_row_fwd", "k_col_fused", "k_row_inv_growth", "k_diffusion", "k_diffusi
        "k_col_adj", "k_row_inv_add", "k_gen_rows", "k_gen_cols")
 
 
+_ASM = {}
+
+
 def _device_asm(src, tmp_path):
+    """gfx950 assembly of one source file (compiled once per test session)."""
+    if src in _ASM:
+        return _ASM[src]
     if not (os.path.exists(HIPCC) or shutil.which(HIPCC)):
         pytest.skip("hipcc not available")
     out = tmp_path / (os.path.basename(src) + ".s")
@@ -29,7 +35,8 @@ def _device_asm(src, tmp_path):
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "--cuda-device-only", "-S",
            os.path.join(PKG, "csrc", src), "-o", str(out)]
     subprocess.run(cmd, check=True, capture_output=True, timeout=600)
-    return out.read_text()
+    _ASM[src] = out.read_text()
+    return _ASM[src]
 
 
 def _kernels(asm):
@@ -50,3 +57,16 @@ def test_hot_kernels_use_global_memory_instructions(src, tmp_path):
     bad = {k: n for k, n in flat.items() if n}
     assert not bad, f"FLAT memory instructions in {bad}"
     assert all(re.search(r"\bglobal_(?:load|store)", b) for b in hot.values())
+
+
+@pytest.mark.parametrize("src", ["step_kernels.hip", "adjoint_kernels.hip"])
+def test_benchmark_size_kernels_do_not_spill(src, tmp_path):
+    """No instance for grid sizes 512..2048 (the batched and single-creature benchmark sizes)
+    spills registers to scratch memory: a spill puts slow scratch loads and stores in the
+    transforms' inner loops (a single one cost the 2048^2 column pass 15 % once)."""
+    kernels = _kernels(_device_asm(src, tmp_path))
+    sized = {k: b for k, b in kernels.items() if re.search(r"ILi(512|1024|2048)E", k)}
+    assert sized, "no sized kernels found"
+    spills = {k: len(re.findall(r"\bscratch_(?:load|store)", b)) for k, b in sized.items()}
+    bad = {k: n for k, n in spills.items() if n}
+    assert not bad, f"scratch spills in {bad}"
