@@ -2,6 +2,11 @@
 
   python cella-nova_amd/prof_summary.py stats <run_kernel_stats.csv>
   python cella-nova_amd/prof_summary.py traffic <fetch run_counter_collection.csv> <write ...csv> [calib MiB] [--json DIR TAG]
+  python cella-nova_amd/prof_summary.py requests <N> <C> <K> <counter csv> [<counter csv> ...]
+
+`requests` averages L1/L2 request counters per kernel launch over any number of counter passes
+and divides each kernel's algorithmic read and write bytes at an N x N grid with C channels and
+K kernels (compact kernel spectra) by its L1-to-L2 read and write requests.
 
 `traffic` averages FETCH_SIZE / WRITE_SIZE (KiB per dispatch) per kernel and scales them by what
 the same counters read for the calibration copy (a device copy of `calib` MiB, dispatched as
@@ -81,9 +86,47 @@ def traffic(fetch_csv, write_csv, calib_mib=256, json_dir=None, tag=None):
                 fh.write("\n")
 
 
+def alg_bytes(n, c, k):
+    """{kernel: (read bytes, write bytes)} per launch: the minimum each kernel must move."""
+    cells = n * n
+    return {"row_fwd_r2c": (4 * c * cells, 4 * c * cells), "col_fused": ((4 * c + k) * cells, 4 * k * cells),
+            "row_inv_growth": (4 * k * cells, 4 * c * cells), "diffusion": (8 * c * cells, 4 * c * cells)}
+
+
+def requests(n, c, k, paths):
+    vals = collections.defaultdict(lambda: collections.defaultdict(list))
+    names = {}
+    for path in paths:
+        for r in csv.DictReader(open(path)):
+            s = short(r["Kernel_Name"])
+            if s:
+                names.setdefault(s, variant(r["Kernel_Name"]))
+                vals[s][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    alg = alg_bytes(n, c, k)
+
+    def mean(s, ctr):
+        v = vals[s].get(ctr) or vals[s].get(ctr + "_sum")
+        return sum(v) / len(v) if v else float("nan")
+
+    print(f"{'kernel':<36} {'rd_req M':>9} {'B/req':>6} {'rd_lat':>7} {'wr_req M':>9} {'B/req':>6} {'wr_lat':>7} "
+          f"{'pend_stall M':>12} {'TLB miss M':>10} {'L2->mem wr M (64B)':>19}")
+    for s in ("row_fwd_r2c", "col_fused", "row_inv_growth", "diffusion"):
+        if s not in vals:
+            continue
+        rd, wr = mean(s, "TCP_TCC_READ_REQ"), mean(s, "TCP_TCC_WRITE_REQ")
+        rl = mean(s, "TCP_TCC_READ_REQ_LATENCY") / rd
+        wl = mean(s, "TCP_TCC_WRITE_REQ_LATENCY") / wr
+        ea, ea64 = mean(s, "TCC_EA0_WRREQ"), mean(s, "TCC_EA0_WRREQ_64B")
+        print(f"{names[s]:<36} {rd / 1e6:>9.3f} {alg[s][0] / rd:>6.0f} {rl:>7.0f} {wr / 1e6:>9.3f} "
+              f"{alg[s][1] / wr:>6.0f} {wl:>7.0f} {mean(s, 'TCP_PENDING_STALL_CYCLES') / 1e6:>12.2f} "
+              f"{mean(s, 'TCP_UTCL1_TRANSLATION_MISS') / 1e6:>10.3f} {ea / 1e6:>10.3f} ({ea64 / 1e6:.3f})")
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "stats":
         stats(sys.argv[2])
+    elif sys.argv[1] == "requests":
+        requests(int(sys.argv[2]), int(sys.argv[3]), int(sys.argv[4]), sys.argv[5:])
     else:
         args = sys.argv[2:]
         jd = tg = None
