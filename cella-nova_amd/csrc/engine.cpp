@@ -1642,7 +1642,8 @@ cn_status cn_adam_step(cn_engine* e, const cn_adam_params* p) {
 // ---------------------------------------------------------------------------------------
 // autotuner: picks, per size class, the FFT geometry (values per thread of the forward and
 // inverse row transforms and of the column transform, column tile width) and the diffusion
-// tile height by timing every candidate on the engine's own buffers with HIP events.
+// tile height by timing every candidate on the engine's own buffers with HIP events, then
+// the combinations of each kernel's fastest candidates as whole steps.
 // ---------------------------------------------------------------------------------------
 }  // extern "C"
 
@@ -1716,18 +1717,38 @@ cn_status tune_class(cn_engine* e, SizeClass& k, uint32_t reps) {
     ti[(i + 1) % rows.size()].push_back(ms[2]);
     td[i % diffs.size()].push_back(ms[3]);
   }
-  auto best = [](const std::vector<std::vector<float>>& t) {
-    size_t b = 0;
-    for (size_t j = 1; j < t.size(); ++j)
-      if (median(t[j]) < median(t[b])) b = j;
-    return b;
+  // Candidates of each kernel, fastest first by their own median.
+  auto ranked = [](const std::vector<std::vector<float>>& t, size_t keep) {
+    std::vector<size_t> idx(t.size());
+    for (size_t j = 0; j < idx.size(); ++j) idx[j] = j;
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return median(t[a]) < median(t[b]); });
+    idx.resize(std::min(keep, idx.size()));
+    return idx;
   };
-  k.fwd_e = rows[best(tf)];
-  k.inv_e = rows[best(ti)];
-  k.col_e = cols[best(tcol)].e;
-  k.col_tc = cols[best(tcol)].tc;
-  k.col_split = cols[best(tcol)].split;
-  k.diff_th = diffs[best(td)];
+  // Joint choice: the kernels of a step share the L2 and run back to back, so the per-kernel
+  // medians only shortlist (3 column geometries, 2 of each other kernel); every combination of
+  // the shortlists is then timed as whole steps, round robin, and the fastest median step wins.
+  const std::vector<size_t> rf = ranked(tf, 2), ri = ranked(ti, 2), rc = ranked(tcol, 3), rd = ranked(td, 2);
+  std::vector<Geom> combos;
+  for (size_t c : rc)
+    for (size_t f : rf)
+      for (size_t i : ri)
+        for (size_t d : rd) combos.push_back({rows[f], rows[i], cols[c].e, cols[c].tc, diffs[d], cols[c].split});
+  std::vector<std::vector<float>> tstep(combos.size());
+  for (uint32_t r = 0; r < reps; ++r)
+    for (size_t j = 0; j < combos.size(); ++j) {
+      float ms[4];
+      const cn_status st = time_class_step(e, k, combos[j], ms);
+      if (st != CN_OK) return st;
+      tstep[j].push_back(ms[0] + ms[1] + ms[2] + ms[3]);
+    }
+  const Geom& g = combos[ranked(tstep, 1)[0]];
+  k.fwd_e = g.fwd_e;
+  k.inv_e = g.inv_e;
+  k.col_e = g.col_e;
+  k.col_tc = g.col_tc;
+  k.col_split = g.col_split;
+  k.diff_th = g.diff_th;
   return CN_OK;
 }
 

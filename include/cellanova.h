@@ -248,8 +248,10 @@ typedef struct cn_plan {
 } cn_plan;
 
 /* Times every candidate geometry of every size class on the device (reps timed launches per
- * candidate, median) and keeps the fastest per kernel.  Synchronises; the current state is
- * unchanged.  Captured graphs are rebuilt on next use. */
+ * candidate, median), shortlists the fastest per kernel (3 column geometries, 2 of each other
+ * kernel), times every combination of the shortlists as whole steps (reps each, median) and
+ * keeps the fastest step.  Synchronises; the current state is unchanged.  Captured graphs are
+ * rebuilt on next use. */
 cn_status cn_autotune(cn_engine* e, uint32_t reps);
 uint32_t cn_engine_size_classes(const cn_engine* e);
 cn_status cn_engine_get_plan(const cn_engine* e, uint32_t size_class, cn_plan* out);
