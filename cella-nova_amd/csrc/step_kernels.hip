@@ -313,6 +313,169 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
 }
 
 // ---------------------------------------------------------------------------------------
+// Row inverse of every kernel's result for two rows at once + growth + weighted sum + exp.
+//
+// Rows y0 and y0 + 1 of a kernel's result are real, so their half spectra Ha, Hb (N/2 complex,
+// slot 0 packing the real bins X[0], X[N/2]) extend to Hermitian spectra Ua, Ub and one
+// N-point complex inverse transform of Y = Ua + i Ub gives y = ua + i ub.  Against two C2R
+// rows (an N/2-point transform and the real split each) this drops the split's twiddle
+// products and pre-pass at the price of one radix-2 stage's worth of transform work.
+//
+// Thread t of a row pair holds Y at kx = t + m T2 (m = 0..E-1, T2 = N/E), i.e. slots m < E/2 at
+// kx < N/2 and slots m >= E/2 at mirrors N - kx = (T2 - t) + (E-1-m) T2 < N/2 of thread T2 - t's
+// first half.  Each thread loads its first-half elements of both rows once and takes the
+// mirrors from LDS:
+//   kx < N/2:  Y = Ha[kx] + i Hb[kx]             = (a.x - b.y, a.y + b.x)
+//   kx > N/2:  Y = conj(Ha[N-kx]) + i conj(Hb[N-kx]) = (a.x + b.y, b.x - a.y)
+//   kx = 0, N/2 (thread 0): Y = (Ha[0].x, Hb[0].x), (Ha[0].y, Hb[0].y).
+// ---------------------------------------------------------------------------------------
+// Where the pair form runs (measured at the benchmark shapes, same build, same run): 2048^2
+// C3K9 inverse-row pass 47.3 us (pairs, 8 values per thread) against 49.4 us (C2R rows, 16
+// values per thread, the faster C2R geometry); at 1024^2 C3K9 the C2R rows are faster (rollout
+// 11.06 against 11.28 ms) and at 512^2 C5K25 the two are on par, so smaller grids keep the C2R
+// rows (a one-wavefront row needs no block barrier; a 2048-point pair at 16 values per thread
+// spans two wavefronts and needs six per kernel).
+template <int N>
+constexpr bool inv_row_pairs() {
+  return N >= 2048;
+}
+
+template <int N, int ER>
+struct PairCfg {
+  static constexpr int E = ER < N ? ER : N;
+  static constexpr int T = N / E;  // threads per row pair
+  static constexpr int PAIRS = T >= 128 ? 1 : ((256 / T) < N / 2 ? (256 / T) : N / 2);
+  static constexpr int THREADS = PAIRS * T;
+  static constexpr int BLOCKS = N / (2 * PAIRS);  // blocks per plane
+  static_assert(N % (2 * PAIRS) == 0, "row pairs per block must divide the plane");
+};
+
+template <int N, int ER, bool WRITE_U>
+__global__ void __launch_bounds__((PairCfg<N, ER>::THREADS)) __attribute__((amdgpu_waves_per_eu(4)))
+k_row_inv_growth_pair(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw, int islot) {
+  using CF = PairCfg<N, ER>;
+  constexpr int L = N / 2, E = CF::E, T = CF::T, PAIRS = CF::PAIRS, H = E / 2;
+  static_assert(E % 2 == 0 && (L / T) == H, "a thread's first-half slots must be E/2");
+  using LM = LdsMap<N, PAIRS, false>;
+  constexpr bool kWave = wave_rows<N, E>();
+  const CreatureDev& d = descs[blockIdx.z];
+  if (d.N != N) return;
+  const int c = blockIdx.y;  // destination channel
+  if (c >= d.C) return;
+  const int pr = threadIdx.x / T, t0 = threadIdx.x % T;
+  const int y0 = (blockIdx.x * PAIRS + pr) * 2;
+  __shared__ float2 lds[LM::size];
+  using TI = TwLds<N, E, true>;
+  __shared__ float2 twl[TI::size + 1];
+  {
+    TwFill<TI, TwNone, CF::THREADS> twf;
+    twf.load(tw, threadIdx.x);
+    twf.store(twl, threadIdx.x);
+  }
+  __syncthreads();
+  const float temp = d.temp;
+  float2 acc[E];  // (row y0, row y0 + 1) of sum_k 2 w_k exp(...) at kx-slot outputs
+#pragma unroll
+  for (int i = 0; i < E; ++i) acc[i] = make_float2(0.f, 0.f);
+  float wsum = 0.f;
+  const int kb = d.dst_off[c], ke = d.dst_off[c + 1];
+  // First-half elements kx = t + m T (m < E/2) of rows y0, y0 + 1 of kernel k's result plane.
+  auto load_own = [&](int k, int t, float2* xa, float2* xb) {
+    const void* plane = (const void*)((const float2*)d.uhat + (size_t)k * N * L);
+    if constexpr (!kColTiled) {
+      const BufRes r = make_res(plane, (uint32_t)N * L * 8u);
+      const uint32_t vo = (uint32_t)(y0 * L + t) * 8u;
+#pragma unroll
+      for (int m = 0; m < H; ++m) {
+        xa[m] = buf_load_c64(r, vo, (uint32_t)(m * T) * 8u);
+        xb[m] = buf_load_c64(r, vo, (uint32_t)(L + m * T) * 8u);
+      }
+    } else {
+      gptr<const float2> X = c64(plane);
+#pragma unroll
+      for (int m = 0; m < H; ++m) {
+        xa[m] = X[res_index(N, y0, t + m * T)];
+        xb[m] = X[res_index(N, y0 + 1, t + m * T)];
+      }
+    }
+  };
+  for (int kk = kb; kk < ke; ++kk) {
+    const int k = d.dst_k[kk];
+    const int t = launder(t0);
+    const float mu = d.mu[k], wk = d.w[k];
+    const float cexp = d.inv_s2[k] * -0.72134752044448170368f;  // -log2(e) / 2 / sigma^2
+    const float w2 = 2.0f * wk;
+    wsum += wk;
+    float2 xa[H], xb[H];
+    __builtin_amdgcn_s_setprio(1);
+    load_own(k, t, xa, xb);
+    __builtin_amdgcn_s_setprio(0);
+    // Mirror exchange: row r's half spectrum at padded position r L + kx of the pair's buffer.
+    xsync<kWave>();
+    if constexpr (T % 16 == 0) {
+      // positions t + c with c a multiple of 16: one base, immediate offsets
+      float2* p = lds + LM::at(t, pr);
+#pragma unroll
+      for (int m = 0; m < H; ++m) {
+        p[LM::off(m * T)] = xa[m];
+        p[LM::off(L + m * T)] = xb[m];
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < H; ++m) {
+        lds[LM::at(t + m * T, pr)] = xa[m];
+        lds[LM::at(L + t + m * T, pr)] = xb[m];
+      }
+    }
+    xsync<kWave>();
+    float2 v[E];
+    static_for<0, E>([&](auto I) {
+      constexpr int i = decltype(I)::value, ci = in_off<N, E, true>(i), m = ci / T;
+      if constexpr (m < H) {
+        const float2 a = xa[m], b = xb[m];
+        const float2 g = make_float2(a.x - b.y, a.y + b.x);
+        if constexpr (m == 0) {
+          v[i] = (t == 0) ? make_float2(a.x, b.x) : g;
+        } else {
+          v[i] = g;
+        }
+      } else {
+        // N - kx = L - t - (m - H) T; thread 0's slot m = H is the packed Nyquist bin.
+        const int q = (L - t - (m - H) * T) & (L - 1);
+        const float2 a = lds[LM::at(q, pr)], b = lds[LM::at(L + q, pr)];
+        const float2 g = make_float2(a.x + b.y, b.x - a.y);
+        if constexpr (m == H) {
+          v[i] = (t == 0) ? make_float2(xa[0].y, xb[0].y) : g;
+        } else {
+          v[i] = g;
+        }
+      }
+    });
+    fft_run<N, E, PAIRS, false, true, kTwN, 0, true>(v, lds, t, pr, twl);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int x = out_pos<N, E, true>(t, i);
+      if constexpr (WRITE_U) {
+        gptr<float> uo = to_global(d.u + (((size_t)islot * d.K + k) * N + y0) * N);
+        uo[x] = v[i].x;
+        uo[N + x] = v[i].y;
+      }
+      const float d0 = v[i].x - mu, d1 = v[i].y - mu;
+      acc[i].x = fmaf(w2, __builtin_amdgcn_exp2f(d0 * (d0 * cexp)), acc[i].x);
+      acc[i].y = fmaf(w2, __builtin_amdgcn_exp2f(d1 * (d1 * cexp)), acc[i].y);
+    }
+  }
+  gptr<float> Ao = to_global(d.A + (((size_t)islot * d.C + c) * N + y0) * N);
+  const int t = launder(t0);
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int x = out_pos<N, E, true>(t, i);
+    Ao[x] = exp_fast(temp * (acc[i].x - wsum));
+    Ao[N + x] = exp_fast(temp * (acc[i].y - wsum));
+  }
+}
+
+// ---------------------------------------------------------------------------------------
 // Diffusion: Z = box3(A) (periodic), new = A * box3(state / Z).  Halo-2 tile in LDS.
 //
 // Mass conservation.  Exactly, sum_p new_p = sum_n (s_n / Z_n) * sum_{p in N(n)} A_p
@@ -484,11 +647,20 @@ static hipError_t launch_rows(const StepLaunch& sl, hipStream_t st, bool fwd) {
     const dim3 g_row((sl.max_c * N + CF::ROWS - 1) / CF::ROWS, 1, sl.count);
     k_row_fwd<N, ER><<<g_row, CF::THREADS, 0, st>>>(sl.descs, sl.sin, sl.tw);
   } else {
-    const dim3 g_inv(CF::BLOCKS, sl.max_c, sl.count);
-    if (sl.write_u)
-      k_row_inv_growth<N, ER, true><<<g_inv, CF::THREADS, 0, st>>>(sl.descs, sl.tw, sl.islot);
-    else
-      k_row_inv_growth<N, ER, false><<<g_inv, CF::THREADS, 0, st>>>(sl.descs, sl.tw, sl.islot);
+    if constexpr (inv_row_pairs<N>()) {
+      using PC = PairCfg<N, ER>;
+      const dim3 g_inv(PC::BLOCKS, sl.max_c, sl.count);
+      if (sl.write_u)
+        k_row_inv_growth_pair<N, ER, true><<<g_inv, PC::THREADS, 0, st>>>(sl.descs, sl.tw, sl.islot);
+      else
+        k_row_inv_growth_pair<N, ER, false><<<g_inv, PC::THREADS, 0, st>>>(sl.descs, sl.tw, sl.islot);
+    } else {
+      const dim3 g_inv(CF::BLOCKS, sl.max_c, sl.count);
+      if (sl.write_u)
+        k_row_inv_growth<N, ER, true><<<g_inv, CF::THREADS, 0, st>>>(sl.descs, sl.tw, sl.islot);
+      else
+        k_row_inv_growth<N, ER, false><<<g_inv, CF::THREADS, 0, st>>>(sl.descs, sl.tw, sl.islot);
+    }
   }
   return hipSuccess;
 }

@@ -120,6 +120,7 @@ inline size_t ct_index(int N, int y, int x) {
 #define CN_COL_TILED 0
 #endif
 constexpr bool kColTiled = CN_COL_TILED != 0;
+
 #ifdef __HIPCC__
 __host__ __device__
 #endif

@@ -20,6 +20,7 @@ import os
 import sys
 
 SHORT = {"k_row_fwd": "row_fwd_r2c", "k_col_fused": "col_fused", "k_row_inv_growth": "row_inv_growth",
+         "k_row_inv_growth_pair": "row_inv_growth",
          "k_diffusion": "diffusion", "k_diffusion_adj": "diffusion_adj", "k_growth_adj_row": "growth_adj_row",
          "k_col_adj": "col_adj", "k_row_inv_add": "row_inv_add", "k_param_reduce": "param_reduce",
          "k_mse_grad": "mse_grad", "k_zero_gparam": "zero_gparam", "k_adam": "adam"}

@@ -309,6 +309,27 @@ def test_full_size_step_vs_fp64(cn, orc, n, R):
     eng.close()
 
 
+@pytest.mark.parametrize("inv_e", [8, 16])
+def test_conv_buffer_2048_row_pairs(cn, orc, inv_e):
+    """At 2048^2 the inverse-row pass transforms two rows per complex FFT; its u-writing form
+    (the convolution readback and the adjoint's recompute) against the numpy FP64 convolution,
+    and the step under both row geometries against FP64."""
+    n = 2048
+    cr = cn.baseline_creature(n, radius=26, seed=3)
+    eng = cn.make_engine(cr)
+    eng.set_plan(n, 16, inv_e, 16, 4, 8)
+    prev = eng.download_all_channels().reshape(3, n, n).astype(np.float64)
+    eng.iterate()
+    conv = eng.download_conv_buffer().reshape(9, n, n) / (n * n)
+    want = orc.conv_numpy_f64(prev, cr["kernels_fft"], cr["src"])
+    err_u = orc.rel_linf(conv, want)
+    got = eng.download_all_channels().reshape(3, n, n)
+    err_s = orc.rel_linf(got, orc.step_numpy_f64(prev, *_args(cr)))
+    print(f"2048^2 C3K9 inv_e={inv_e}: conv readback vs FP64 {err_u:.2e}, step vs FP64 {err_s:.2e}")
+    assert err_u <= 1e-6 and err_s <= TOL
+    eng.close()
+
+
 def test_full_size_mass_2048(cn):
     cr = cn.baseline_creature(2048, radius=26, seed=0)
     eng = cn.make_engine(cr)
