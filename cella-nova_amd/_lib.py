@@ -136,6 +136,15 @@ SIGNATURES = {
     "cn_creature_get_loss": (c_int, [c_void_p, c_uint32, POINTER(c_double)]),
     "cn_creature_get_grads": (c_int, [c_void_p, c_uint32, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p,
                                       c_size_t, c_int]),
+    # audio consumer of the rendered frames
+    "cn_creature_render_features": (c_int, [c_void_p, c_uint32, c_void_p, c_void_p]),
+    "cn_render_features": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "cn_audio_extract_features": (c_int, [c_void_p, c_uint32, c_void_p, c_void_p]),
+    "cn_audio_create": (c_int, [c_uint32, POINTER(c_void_p)]),
+    "cn_audio_destroy": (c_int, [c_void_p]),
+    "cn_audio_generate_frame": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_void_p, c_size_t]),
+    "cn_audio_activity": (c_int, [c_void_p, POINTER(c_float)]),
+    "cn_write_wav": (c_int, [c_char_p, c_void_p, c_size_t, c_uint32, ctypes.c_uint16]),
 }
 
 _lib = None

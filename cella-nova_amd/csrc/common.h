@@ -50,6 +50,11 @@ inline bool finite_all(const float* p, size_t n) {
 // Synthetic creatures (creature.cpp): parameter checks and growth parameters of the
 // benchmark distribution, shared by the host and device generators.
 cn_status check_creature_params(const cn_creature_params* p);
+
+// Audio features (means and variances of bytes / 255 per RGB channel) from exact byte sums over
+// `cells` pixels (audio.cpp).
+void features_from_sums(const uint64_t s1[3], const uint64_t s2[3], uint64_t cells, float means[3],
+                        float variances[3]);
 void generate_growth(const cn_creature_params* p, float* mu, float* sigma, float* weights);
 
 }  // namespace cn

@@ -239,6 +239,7 @@ struct cn_engine {
   float2* tw = nullptr;
   double2* mass_partial = nullptr;
   int mass_nblk = kMassBlocks;
+  unsigned long long* stats_partial = nullptr;  // frame statistics (kStatsBlocks x 6), on first use
   std::vector<void*> allocs;
   hipGraphExec_t step_graph[2] = {nullptr, nullptr};
   hipGraphExec_t render_graph[2] = {nullptr, nullptr};  // step + render epilogue
@@ -277,6 +278,18 @@ cn_status dalloc(cn_engine* e, std::vector<void*>& owner, void** p, size_t bytes
   e->bytes += bytes;
   *p = q;
   return CN_OK;
+}
+
+// Releases an allocation made by dalloc (`bytes` is its size); a null pointer is ignored.
+// hipFree of a live allocation has no recoverable failure, so its status is not propagated:
+// the buffer leaves the owner list and the byte count together with the memory.
+void dfree(cn_engine* e, std::vector<void*>& owner, void* p, size_t bytes) {
+  if (!p) return;
+  auto it = std::find(owner.begin(), owner.end(), p);
+  if (it == owner.end()) return;
+  owner.erase(it);
+  e->bytes -= bytes;
+  (void)hipFree(p);
 }
 
 void drop_graphs(cn_engine* e) {
@@ -338,6 +351,30 @@ cn_status upload_desc(cn_engine* e, int i) {
                         e->stream));
   CN_HIP(hipStreamSynchronize(e->stream));
   return CN_OK;
+}
+
+// Uploads creature i's host descriptor after a change to its buffers or flags (not to its
+// growth parameters).  If that fails, the host copy goes back to `before` (what the device
+// held; growth fields as the upload's read-back left them) and, unless the failure came before
+// the copy was issued, is copied to the device again outside the checked wrapper, since the
+// failed copy may still land.  *restored tells whether the device copy is then known to equal
+// the host's: only then may the caller free buffers that the changed descriptor named.
+cn_status commit_desc(cn_engine* e, int i, const CreatureDev& before, bool* restored) {
+  const cn_status st = upload_desc(e, i);
+  *restored = true;
+  if (st == CN_OK) return CN_OK;
+  Creature& c = e->cr[i];
+  const CreatureDev now = c.dev;
+  c.dev = before;
+  std::memcpy(c.dev.mu, now.mu, sizeof now.mu);
+  std::memcpy(c.dev.sigma, now.sigma, sizeof now.sigma);
+  std::memcpy(c.dev.inv_s2, now.inv_s2, sizeof now.inv_s2);
+  std::memcpy(c.dev.w, now.w, sizeof now.w);
+  if (!e->growth_on_device)  // else the failure was the growth read-back: nothing was copied
+    *restored = hipMemcpyAsync(e->cls[c.cls].d_descs + c.idx, &c.dev, sizeof(CreatureDev), hipMemcpyHostToDevice,
+                               e->stream) == hipSuccess &&
+                hipStreamSynchronize(e->stream) == hipSuccess;
+  return st;
 }
 
 StepLaunch class_launch(cn_engine* e, const SizeClass& k, int sin, int sout) {
@@ -457,28 +494,29 @@ cn_status ensure_slots(cn_engine* e, int want) {
     // A creature already grown by an earlier call that failed part-way keeps its buffer.
     if (c.dev.nslots >= want) continue;
     void* p = nullptr;
-    const cn_status st = dalloc(e, c.allocs, &p, (size_t)want * c.slot_elems() * 4);
+    const size_t bytes = (size_t)want * c.slot_elems() * 4;
+    cn_status st = dalloc(e, c.allocs, &p, bytes);
     if (st != CN_OK) return st;
-    CN_HIP(hipMemcpyAsync(p, c.dev.state, 2 * c.slot_elems() * 4, hipMemcpyDeviceToDevice, e->stream));
-    CN_HIP(hipStreamSynchronize(e->stream));
+    st = [&]() -> cn_status {
+      CN_HIP(hipMemcpyAsync(p, c.dev.state, 2 * c.slot_elems() * 4, hipMemcpyDeviceToDevice, e->stream));
+      CN_HIP(hipStreamSynchronize(e->stream));
+      return CN_OK;
+    }();
+    if (st != CN_OK) {  // no descriptor names the new buffer yet
+      dfree(e, c.allocs, p, bytes);
+      return st;
+    }
     // New buffer into the descriptor first; the old one is freed only once the device copy
     // of the descriptor no longer refers to it.
-    float* const old = c.dev.state;
-    const int old_n = c.dev.nslots;
+    const CreatureDev before = c.dev;
     c.dev.state = (float*)p;
     c.dev.nslots = want;
-    const cn_status s2 = upload_desc(e, (int)i);
-    if (s2 != CN_OK) {
-      c.dev.state = old;
-      c.dev.nslots = old_n;
-      return s2;
+    bool restored = false;
+    if ((st = commit_desc(e, (int)i, before, &restored)) != CN_OK) {
+      if (restored) dfree(e, c.allocs, p, bytes);
+      return st;
     }
-    auto it = std::find(c.allocs.begin(), c.allocs.end(), (void*)old);
-    if (it != c.allocs.end()) {
-      CN_HIP(hipFree(*it));
-      e->bytes -= (size_t)old_n * c.slot_elems() * 4;  // the size of the buffer just freed
-      c.allocs.erase(it);
-    }
+    dfree(e, c.allocs, before.state, (size_t)before.nslots * c.slot_elems() * 4);
   }
   e->nslots = want;
   return CN_OK;
@@ -512,8 +550,9 @@ cn_status ensure_adjoint(cn_engine* e) {
     if ((st = need(d.gparam, 6 * K * 4)) != CN_OK) return st;
     if ((st = need(d.target, C * P * 4)) != CN_OK) return st;
     if ((st = need(d.lossp, (size_t)kLossPartials * 4)) != CN_OK) return st;
-    c.adj = true;
+    // The buffers stay with the creature whatever happens to the upload; a retry uploads again.
     if ((st = upload_desc(e, (int)i)) != CN_OK) return st;
+    c.adj = true;
   }
   return CN_OK;
 }
@@ -536,26 +575,28 @@ cn_status ensure_islots(cn_engine* e, int want) {
     void *pa = nullptr, *pu = nullptr;
     cn_status st = dalloc(e, c.allocs, &pa, a_bytes);
     if (st != CN_OK) return st;
-    if ((st = dalloc(e, c.allocs, &pu, u_bytes)) != CN_OK) return st;
-    float* old[2] = {c.dev.A, c.dev.u};
-    const size_t old_bytes[2] = {(size_t)c.islots * c.C * c.plane() * 4, (size_t)c.islots * c.K * c.plane() * 4};
+    // On a failure the new arrays are released again, so a retry neither keeps nor counts
+    // them twice.
+    if ((st = dalloc(e, c.allocs, &pu, u_bytes)) != CN_OK) {
+      dfree(e, c.allocs, pa, a_bytes);
+      return st;
+    }
+    const CreatureDev before = c.dev;
+    const size_t old_bytes[2] = {(size_t)c.islots * c.C * c.plane() * 4,
+                                 c.dev.u ? (size_t)c.islots * c.K * c.plane() * 4 : 0};
     c.dev.A = (float*)pa;
     c.dev.u = (float*)pu;
-    if ((st = upload_desc(e, (int)i)) != CN_OK) {
-      c.dev.A = old[0];
-      c.dev.u = old[1];
+    bool restored = false;
+    if ((st = commit_desc(e, (int)i, before, &restored)) != CN_OK) {
+      if (restored) {
+        dfree(e, c.allocs, pa, a_bytes);
+        dfree(e, c.allocs, pu, u_bytes);
+      }
       return st;
     }
     c.islots = n;
-    for (int j = 0; j < 2; ++j) {
-      if (!old[j]) continue;
-      auto it = std::find(c.allocs.begin(), c.allocs.end(), (void*)old[j]);
-      if (it != c.allocs.end()) {
-        CN_HIP(hipFree(*it));
-        e->bytes -= old_bytes[j];
-        c.allocs.erase(it);
-      }
-    }
+    dfree(e, c.allocs, before.A, old_bytes[0]);
+    dfree(e, c.allocs, before.u, old_bytes[1]);
   }
   return CN_OK;
 }
@@ -854,8 +895,10 @@ cn_status cn_creature_set_kernel(cn_engine* e, uint32_t i, uint32_t k, const flo
                           hipMemcpyHostToDevice, e->stream));
   CN_HIP(hipStreamSynchronize(e->stream));
   if (c.dev.ksym[k] != (sym ? 1 : 0)) {
+    const CreatureDev before = c.dev;
     c.dev.ksym[k] = sym ? 1 : 0;
-    return upload_desc(e, (int)i);
+    bool restored = false;
+    return commit_desc(e, (int)i, before, &restored);
   }
   return CN_OK;
   CN_CATCH
@@ -1032,20 +1075,23 @@ cn_status cn_run_steps(cn_engine* e, uint64_t n) {
   CN_HIP(hipSetDevice(e->device));
   // Runs of kMultiSteps steps replay one graph holding all of them (an even count, so the
   // ping-pong parity is back where it started): one graph launch per run instead of one per
-  // step, which saves the launch boundary (about 4.5 us per 2048^2 step, 2.3 %).
+  // step, which saves the launch boundary (about 4.5 us per 2048^2 step, 2.3 %).  Both
+  // parities' graphs are captured by the first call, whatever its step count, so that later
+  // calls (a timed run after a short warm-up, say) only launch.
+  for (int par = 0; par < 2; ++par) {
+    if (e->multi_graph[par]) continue;
+    const cn_status st = capture(e, &e->multi_graph[par], [&] {
+      for (int j = 0; j < kMultiSteps; ++j) {
+        const int a = par ^ (j & 1);
+        const hipError_t err = launch_all_steps(e, a, a ^ 1);
+        if (err != hipSuccess) return err;
+      }
+      return hipSuccess;
+    });
+    if (st != CN_OK) return st;
+  }
   while (n >= (uint64_t)kMultiSteps) {
     const int cur = e->cur;
-    if (!e->multi_graph[cur]) {
-      const cn_status st = capture(e, &e->multi_graph[cur], [&] {
-        for (int j = 0; j < kMultiSteps; ++j) {
-          const int a = cur ^ (j & 1);
-          const hipError_t err = launch_all_steps(e, a, a ^ 1);
-          if (err != hipSuccess) return err;
-        }
-        return hipSuccess;
-      });
-      if (st != CN_OK) return st;
-    }
     CN_HIP(hipGraphLaunch(e->multi_graph[cur], e->stream));
     e->steps += kMultiSteps;
     e->have_prev = true;
@@ -1109,9 +1155,14 @@ static cn_status ensure_render(cn_engine* e) {
     void* p = nullptr;
     const cn_status st = dalloc(e, c.allocs, &p, c.plane() * 4);
     if (st != CN_OK) return st;
+    const CreatureDev before = c.dev;
     c.dev.render = (unsigned int*)p;
-    const cn_status su = upload_desc(e, (int)i);
-    if (su != CN_OK) return su;
+    bool restored = false;
+    const cn_status su = commit_desc(e, (int)i, before, &restored);
+    if (su != CN_OK) {
+      if (restored) dfree(e, c.allocs, p, c.plane() * 4);
+      return su;
+    }
   }
   return CN_OK;
 }
@@ -1156,6 +1207,43 @@ cn_status cn_creature_download_render(cn_engine* e, uint32_t i, uint8_t* rgb, si
 cn_status cn_download_render(cn_engine* e, uint8_t* rgb, size_t n) {
   CN_TRY
   return cn_creature_download_render(e, 0, rgb, n);
+  CN_CATCH
+}
+
+cn_status cn_creature_render_features(cn_engine* e, uint32_t i, float* means, float* variances) {
+  CN_TRY
+  clear_error();
+  cn_status st = check_creature(e, i);
+  if (st != CN_OK) return st;
+  if (!means || !variances) return set_error(CN_ERR_INVALID_ARG, "null argument");
+  const Creature& c = e->cr[i];
+  const uint64_t cells = c.plane();
+  uint64_t s1[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
+  if (c.dev.render) {  // before the first rendered frame the frame is all zeros
+    CN_HIP(hipSetDevice(e->device));
+    if (!e->stats_partial) {
+      void* p = nullptr;
+      if ((st = dalloc(e, e->allocs, &p, (size_t)kStatsBlocks * 6 * 8)) != CN_OK) return st;
+      e->stats_partial = (unsigned long long*)p;
+    }
+    CN_HIP(launch_render_stats(c.dev.render, cells, e->stats_partial, e->stream));
+    std::vector<unsigned long long> part((size_t)kStatsBlocks * 6);
+    CN_HIP(hipMemcpyAsync(part.data(), e->stats_partial, part.size() * 8, hipMemcpyDeviceToHost, e->stream));
+    CN_HIP(hipStreamSynchronize(e->stream));
+    for (int b = 0; b < kStatsBlocks; ++b)
+      for (int ch = 0; ch < 3; ++ch) {
+        s1[ch] += part[(size_t)6 * b + ch];
+        s2[ch] += part[(size_t)6 * b + 3 + ch];
+      }
+  }
+  features_from_sums(s1, s2, cells, means, variances);
+  return CN_OK;
+  CN_CATCH
+}
+
+cn_status cn_render_features(cn_engine* e, float* means, float* variances) {
+  CN_TRY
+  return cn_creature_render_features(e, 0, means, variances);
   CN_CATCH
 }
 
@@ -1255,8 +1343,13 @@ cn_status cn_download_conv_real(cn_engine* e, float* out, size_t n) {
   if (!c.dev.u) {
     void* p = nullptr;
     if ((st = dalloc(e, c.allocs, &p, tot * 4)) != CN_OK) return st;
+    const CreatureDev before = c.dev;
     c.dev.u = (float*)p;
-    if ((st = upload_desc(e, 0)) != CN_OK) return st;
+    bool restored = false;
+    if ((st = commit_desc(e, 0, before, &restored)) != CN_OK) {
+      if (restored) dfree(e, c.allocs, p, tot * 4);
+      return st;
+    }
   }
   // Recompute the convolution of the state the most recent step started from, for creature 0
   // alone: in a batch the other creatures may have no u buffer.

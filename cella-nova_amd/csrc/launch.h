@@ -61,4 +61,10 @@ hipError_t launch_gather_grads(const CreatureDev* descs, int count, float* out, 
 hipError_t launch_mass(const CreatureDev* descs, int count, int slot, int max_c, double2* partial, int nblk,
                        hipStream_t st);
 
+// Per-channel byte sums of a packed 0x00RRGGBB frame of `cells` pixels (a multiple of 4):
+// kStatsBlocks blocks each write [sum r, g, b, sum r^2, g^2, b^2] to partial[6 * block ...].
+constexpr int kStatsThreads = 256;
+constexpr int kStatsBlocks = 256;
+hipError_t launch_render_stats(const uint32_t* frame, size_t cells, unsigned long long* partial, hipStream_t st);
+
 }  // namespace cn

@@ -181,6 +181,14 @@ class _Engine:
         check(lib().cn_creature_download_render(self._h, int(i), out.ctypes.data, out.size))
         return out
 
+    def creature_render_features(self, i):
+        """(means, variances), float32[3] each: the audio features of creature i's last rendered
+        frame, reduced on the device (audio.rs:181-209)."""
+        means = np.empty(3, dtype=np.float32)
+        variances = np.empty(3, dtype=np.float32)
+        check(lib().cn_creature_render_features(self._h, int(i), means.ctypes.data, variances.ctypes.data))
+        return means, variances
+
     # --- all creatures -------------------------------------------------------------------------
     def iterate(self):
         check(lib().cn_iterate(self._h))
@@ -339,6 +347,10 @@ class GpuMaceLenia(_Engine):
     def download_render(self):
         """RGB24 bytes of the last rendered frame, N*N*3 uint8 (mod.rs:302-335)."""
         return self.creature_download_render(0).reshape(-1)
+
+    def render_features(self):
+        """Audio features of the last rendered frame, computed on the device."""
+        return self.creature_render_features(0)
 
     def mass_fp64(self):
         """Per-channel FP64 compensated mass of the current state."""

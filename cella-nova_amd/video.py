@@ -1,4 +1,4 @@
-"""Video command line of the reference (ml-rs/main.rs:170-365), without its audio consumer.
+"""Video command line of the reference (ml-rs/main.rs:170-365), with its audio consumer.
 
 For every creature (--creature NAME, or every seed/*.json as main.rs:128-168 discovers them): load
 seed/NAME.json and kernels/NAME_<size>.bin with size = seed_size rounded up to a power of two
@@ -7,7 +7,11 @@ seed/NAME.json and kernels/NAME_<size>.bin with size = seed_size rounded up to a
 reference's encoder arguments (main.rs:238-264) when an `ffmpeg` executable is on PATH, otherwise
 to <output>/<NAME>.rgb (raw rgb24, the same bytes ffmpeg would have read; the command that
 encodes it is printed).  An existing <output>/<NAME>.mp4 is skipped as in main.rs:203-207.
---with-sound (audio.rs) is not part of this build and is refused.
+--with-sound (main.rs:219-235, 286-296, 315-361): one AudioSynth per video at 44.1 kHz, fed per
+frame with the frame's features (reduced on the device from the packed frame instead of the
+downloaded bytes; the same values) and seconds * 44100 / frames samples, the remainder spread
+over the first frames; the samples go to <output>/<NAME>.wav, which ffmpeg muxes into the video
+(AAC, -shortest) and which is then removed; without ffmpeg the WAV stays beside the raw frames.
 """
 import argparse
 import os
@@ -48,7 +52,13 @@ def ffmpeg_args(size, fps, path):
             "-pix_fmt", "yuv420p", path]
 
 
-def render_video(name, seconds=60, fps=60, output="videos", temp=1.0, root=".", use_ffmpeg=None, log=print):
+def mux_args(video, wav, out):
+    """The reference's audio mux invocation (main.rs:329-345)."""
+    return ["ffmpeg", "-y", "-i", video, "-i", wav, "-c:v", "copy", "-c:a", "aac", "-shortest", out]
+
+
+def render_video(name, seconds=60, fps=60, output="videos", temp=1.0, root=".", use_ffmpeg=None, log=print,
+                 with_sound=False):
     """Renders one creature; returns the path written (None when the video already exists)."""
     mp4 = os.path.join(output, f"{name}.mp4")
     if os.path.exists(mp4):
@@ -60,6 +70,13 @@ def render_video(name, seconds=60, fps=60, output="videos", temp=1.0, root=".", 
     eng, size = setup_simulation(name, temp, root)
     total = int(seconds) * int(fps)
     log(f"Generating {total} frames at {fps} FPS ({seconds} seconds)")
+    synth, chunks = None, []
+    if with_sound:
+        from .audio import SAMPLE_RATE, AudioSynth
+        synth = AudioSynth(SAMPLE_RATE)
+        total_samples = int(seconds) * SAMPLE_RATE
+        per_frame, remainder = total_samples // total, total_samples % total
+        log(f"Audio enabled: {SAMPLE_RATE} Hz, {per_frame} samples/frame (+{remainder} remainder)")
     proc, sink, path = None, None, mp4
     try:
         if use_ffmpeg:
@@ -73,10 +90,14 @@ def render_video(name, seconds=60, fps=60, output="videos", temp=1.0, root=".", 
             if step % int(fps) == 0:
                 log(f"Frame {step + 1}/{total} ({step // int(fps)}s)...")
             sink.write(eng.download_render().tobytes())
+            if synth is not None:
+                chunks.append(synth.generate_frame(eng.render_features(), per_frame + (1 if step < remainder else 0)))
     finally:
         if sink is not None:
             sink.close()
         eng.close()
+        if synth is not None:
+            synth.close()
     if proc is not None:
         if proc.wait() != 0:
             raise RuntimeError(f"ffmpeg encoding failed (exit code {proc.returncode})")
@@ -84,6 +105,21 @@ def render_video(name, seconds=60, fps=60, output="videos", temp=1.0, root=".", 
     else:
         log(f"Raw rgb24 frames saved to: {path} (no ffmpeg on PATH; encode with: "
             f"{' '.join(ffmpeg_args(size, fps, mp4)).replace(' -i - ', f' -i {path} ')})")
+    if with_sound and chunks:
+        import numpy as np
+        from .audio import SAMPLE_RATE, write_wav
+        wav = os.path.join(output, f"{name}.wav")
+        samples = np.concatenate(chunks)
+        write_wav(wav, samples, SAMPLE_RATE, 2)
+        log(f"Audio: {samples.size / (SAMPLE_RATE * 2.0):.1f}s of stereo PCM in {wav}")
+        if proc is not None:
+            muxed = os.path.join(output, f"{name}_muxed.mp4")
+            if subprocess.run(mux_args(mp4, wav, muxed)).returncode == 0:
+                os.replace(muxed, mp4)
+                os.remove(wav)
+                log(f"Audio muxed into: {mp4}")
+            else:
+                log(f"ffmpeg muxing failed. WAV kept at: {wav}")
     return path
 
 
@@ -96,9 +132,6 @@ def _main(argv=None):
     ap.add_argument("--temp", type=float, default=1.0)
     ap.add_argument("--with-sound", action="store_true")
     args = ap.parse_args(argv)
-    if args.with_sound:
-        print("--with-sound: the audio consumer (audio.rs) is not part of this build", file=sys.stderr)
-        return 2
     if args.seconds <= 0 or args.fps <= 0:
         print("--seconds and --fps must be positive", file=sys.stderr)
         return 2
@@ -110,7 +143,7 @@ def _main(argv=None):
         print(f"Found {len(names)} creatures in seed/. Generating videos for all...")
     for name in names:
         print(f"\n  Creature: {name}")
-        render_video(name, args.seconds, args.fps, args.output, args.temp)
+        render_video(name, args.seconds, args.fps, args.output, args.temp, with_sound=args.with_sound)
     return 0
 
 

@@ -351,6 +351,35 @@ cn_status cn_creature_generate(cn_engine* e, uint32_t i, const cn_creature_param
 cn_status cn_creature_download_kernels(cn_engine* e, uint32_t i, float* out, size_t n_floats);
 
 /* ------------------------------------------------------------------------------------ */
+/* Audio consumer of the rendered frames (ml-rs/audio.rs, used by main.rs:219-360)         */
+/* ------------------------------------------------------------------------------------ */
+
+/* AudioSynth::extract_features (audio.rs:181-209) on the device: per RGB channel, the mean and
+ * the variance (max(mean of squares - mean^2, 0), f32) of byte / 255 over the last frame that
+ * cn_iterate_and_render packed for creature i, from exact integer byte sums (the reference sums
+ * byte / 255 in f64 over the downloaded RGB24 bytes).  All zeros before the first rendered
+ * frame.  Synchronises. */
+cn_status cn_creature_render_features(cn_engine* e, uint32_t creature, float means[3], float variances[3]);
+cn_status cn_render_features(cn_engine* e, float means[3], float variances[3]);
+/* The same features of host RGB24 bytes (3 * grid^2 bytes, row major), as extract_features
+ * computes them from `render_rgb` and `grid_size`. */
+cn_status cn_audio_extract_features(const uint8_t* rgb, uint32_t grid, float means[3], float variances[3]);
+
+/* AudioSynth (audio.rs:104-160): the sonar-ping synthesiser, one per video. */
+typedef struct cn_audio cn_audio;
+cn_status cn_audio_create(uint32_t sample_rate, cn_audio** out); /* AudioSynth::new */
+cn_status cn_audio_destroy(cn_audio* a);
+/* AudioSynth::generate_frame (audio.rs:310-396): triggers pings from the features of one video
+ * frame, then synthesises num_samples stereo samples into out (n = 2 * num_samples int16,
+ * interleaved left, right). */
+cn_status cn_audio_generate_frame(cn_audio* a, const float means[3], const float variances[3],
+                                  uint32_t num_samples, int16_t* out, size_t n);
+cn_status cn_audio_activity(const cn_audio* a, float* out); /* AudioSynth::activity_level */
+/* write_wav (audio.rs:404-441): 16-bit PCM RIFF/WAVE file of `count` interleaved samples. */
+cn_status cn_write_wav(const char* path, const int16_t* samples, size_t count, uint32_t sample_rate,
+                       uint16_t channels);
+
+/* ------------------------------------------------------------------------------------ */
 /* Test hooks                                                                            */
 /* ------------------------------------------------------------------------------------ */
 

@@ -53,6 +53,22 @@ def test_step_survives_each_injected_failure(cn):
     assert n >= 2  # hipSetDevice and the capture at least
 
 
+def test_render_survives_each_injected_failure(cn):
+    cr = cn.baseline_creature(32, radius=4, seed=34, patch=16)
+    ref = cn.make_engine(cr)
+    ref.iterate_and_render()
+    want = (ref.download_all_channels(), ref.download_render(), ref.info()["device_bytes"])
+    ref.close()
+
+    def same(e):
+        assert np.array_equal(e.download_all_channels(), want[0])
+        assert np.array_equal(e.download_render(), want[1])
+        assert e.info()["device_bytes"] == want[2]
+
+    n = _walk(cn, lambda: cn.make_engine(cr), lambda e: cn.lib().cn_iterate_and_render(e._h), same)
+    assert n >= 3  # the frame buffer, its descriptor upload and the capture at least
+
+
 def test_rollout_survives_each_injected_failure(cn):
     cr = cn.baseline_creature(32, radius=4, seed=32, patch=16)
     target = cn.baseline_creature(32, radius=4, seed=33, patch=16)["state"]
@@ -60,6 +76,7 @@ def test_rollout_survives_each_injected_failure(cn):
     ref.rollout_grad(6, 2)
     want = ref.creature_grads(0)
     want_loss = ref.creature_loss(0)
+    want_bytes = ref.info()["device_bytes"]
     ref.close()
 
     def same(e):
@@ -67,6 +84,8 @@ def test_rollout_survives_each_injected_failure(cn):
         for k in want:
             assert np.array_equal(g[k], want[k]), k
         assert e.creature_loss(0) == want_loss
+        # Buffers of a failed attempt are neither kept twice nor counted twice.
+        assert e.info()["device_bytes"] == want_bytes
 
     n = _walk(cn, lambda: _engine(cn, cr, target), lambda e: cn.lib().cn_rollout_grad(e._h, 6, 2), same)
     assert n >= 10  # state and intermediate slots, two captures, the launch

@@ -3,10 +3,14 @@ the reference's encoder arguments.  Frame content is checked on the GPU (test_gp
 import os
 
 
-def test_refuses_sound_and_bad_rates(cn, capsys):
-    assert cn.video._main(["--with-sound", "--creature", "x"]) == 2
-    assert "audio" in capsys.readouterr().err
+def test_refuses_bad_rates(cn):
     assert cn.video._main(["--creature", "x", "--fps", "0"]) == 2
+    assert cn.video._main(["--with-sound", "--creature", "x", "--seconds", "0"]) == 2
+
+
+def test_mux_arguments_match_the_reference(cn):
+    assert cn.video.mux_args("v/x.mp4", "v/x.wav", "v/x_muxed.mp4") == [
+        "ffmpeg", "-y", "-i", "v/x.mp4", "-i", "v/x.wav", "-c:v", "copy", "-c:a", "aac", "-shortest", "v/x_muxed.mp4"]
 
 
 def test_no_creatures_found(cn, tmp_path, monkeypatch, capsys):
