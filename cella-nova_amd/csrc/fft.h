@@ -271,11 +271,39 @@ struct TwFill {
   }
 };
 
+// The twiddles of a plan's last pass when it uses two LDS factors (W = A B, see TwLds), formed
+// once per thread and kept in registers: a kernel that runs the same plan for many rows or
+// kernels, and has registers to spare, saves the two LDS reads and the complex product of
+// every such twiddle on every transform (the values are the same products, bit for bit).
+template <int L, int E, bool INV>
+struct TwRegs {
+  using PL = Plan<L, E>;
+  using TL = TwLds<L, E, INV>;
+  static constexpr int P = PL::npass - 1;
+  static constexpr bool kOn = P >= 1 && !TL::direct(P);
+  static constexpr int R = PL::radix(P, INV), NS = PL::span(P, INV), NB = E / R;
+  float2 w[kOn ? NB * (R - 1) : 1];
+  // twl: the TwLds<L, E, INV> block in LDS (filled and ordered by a barrier before the call).
+  __device__ __forceinline__ void init(const float2* twl, int t) {
+    if constexpr (kOn) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int k = (t + b * PL::T) & (NS - 1);
+        const float2* tb = twl + TL::off(P) + (k & 15);
+        const float2* ta = twl + TL::off(P) + (R - 1) * 16 + (k >> 4);
+#pragma unroll
+        for (int m = 1; m < R; ++m) w[b * (R - 1) + m - 1] = c_mul(ta[(m - 1) * (NS / 16)], tb[(m - 1) * 16]);
+      }
+    }
+  }
+};
+
 // One Stockham pass on register data (twiddle + radix-R butterflies) for thread t.  Pass P of
 // the plan; LT: the twiddles come from TwLds<L, E, INV> at `tw` (LDS), else from the global
-// table.
-template <int L, int E, int R, int NS, bool INV, int TWN, int P, bool LT>
-__device__ __forceinline__ void pass_compute(float2* v, int t, const float2* __restrict__ tw) {
+// table; REG: the last pass's two-factor twiddles come from a TwRegs at `wr`.
+template <int L, int E, int R, int NS, bool INV, int TWN, int P, bool LT, bool REG = false>
+__device__ __forceinline__ void pass_compute(float2* v, int t, const float2* __restrict__ tw,
+                                             const float2* wr = nullptr) {
   constexpr int T = L / E;
   constexpr int NB = E / R;
 #pragma unroll
@@ -291,6 +319,12 @@ __device__ __forceinline__ void pass_compute(float2* v, int t, const float2* __r
 #pragma unroll
           for (int m = 1; m < R; ++m) {
             const float2 w = twp[(m - 1) * NS];
+            v[b * R + m] = INV ? c_mulc(v[b * R + m], w) : c_mul(v[b * R + m], w);
+          }
+        } else if constexpr (REG && P == TwRegs<L, E, INV>::P) {
+#pragma unroll
+          for (int m = 1; m < R; ++m) {
+            const float2 w = wr[b * (R - 1) + m - 1];
             v[b * R + m] = INV ? c_mulc(v[b * R + m], w) : c_mul(v[b * R + m], w);
           }
         } else {
@@ -352,14 +386,14 @@ __device__ __forceinline__ int launder(int x) {
 // LdsMap<L,NT,IL>::size elements; every exchange is bracketed by barriers (wavefront-level for
 // row-layout transforms that fit in a wavefront, block-level otherwise), so the routine can be
 // called repeatedly on the same buffer.
-template <int L, int E, int NT, bool IL, bool INV, int TWN, int P = 0, bool LT = false>
+template <int L, int E, int NT, bool IL, bool INV, int TWN, int P = 0, bool LT = false, bool REG = false>
 __device__ __forceinline__ void fft_run(float2* v, float2* lds, int t, int lane,
-                                        const float2* __restrict__ tw) {
+                                        const float2* __restrict__ tw, const float2* wr = nullptr) {
   using PL = Plan<L, E>;
   constexpr int R = PL::radix(P, INV);
   constexpr int NS = PL::span(P, INV);
   t = launder(t);
-  pass_compute<L, E, R, NS, INV, TWN, P, LT>(v, t, tw);
+  pass_compute<L, E, R, NS, INV, TWN, P, LT, REG>(v, t, tw, wr);
   if constexpr (P + 1 < PL::npass) {
     constexpr int T = L / E;
     constexpr int R2 = PL::radix(P + 1, INV);
@@ -398,7 +432,7 @@ __device__ __forceinline__ void fft_run(float2* v, float2* lds, int t, int lane,
         for (int m = 0; m < R2; ++m) v[b * R2 + m] = lds[LM::at(j + m * (L / R2), lane)];
       }
     }
-    fft_run<L, E, NT, IL, INV, TWN, P + 1, LT>(v, lds, t, lane, tw);
+    fft_run<L, E, NT, IL, INV, TWN, P + 1, LT, REG>(v, lds, t, lane, tw, wr);
   }
 }
 

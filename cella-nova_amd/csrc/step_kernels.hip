@@ -373,6 +373,10 @@ k_row_inv_growth_pair(const CreatureDev* __restrict__ descs, const float2* __res
     twf.store(twl, threadIdx.x);
   }
   __syncthreads();
+  // The last pass's two-factor twiddles are the same for every kernel of the destination: form
+  // them once (the pass kernel's occupancy is bound by its LDS, which leaves these registers).
+  TwRegs<N, E, true> twr;
+  twr.init(twl, launder(t0));
   const float temp = d.temp;
   float2 acc[E];  // (row y0, row y0 + 1) of sum_k 2 w_k exp(...) at kx-slot outputs
 #pragma unroll
@@ -399,9 +403,14 @@ k_row_inv_growth_pair(const CreatureDev* __restrict__ descs, const float2* __res
       }
     }
   };
+  // At 8 values per thread (without the u stores) the kernel-loop-invariant index arithmetic
+  // fits the registers the LDS-bound occupancy leaves (71 VGPRs, 10 % fewer loop instructions);
+  // otherwise the lane index is laundered per kernel so that it is recomputed instead of spilled.
+  constexpr bool kHoist = E <= 8 && !WRITE_U;
+  const int tl = launder(t0);
   for (int kk = kb; kk < ke; ++kk) {
     const int k = d.dst_k[kk];
-    const int t = launder(t0);
+    const int t = kHoist ? tl : launder(t0);
     const float mu = d.mu[k], wk = d.w[k];
     const float cexp = d.inv_s2[k] * -0.72134752044448170368f;  // -log2(e) / 2 / sigma^2
     const float w2 = 2.0f * wk;
@@ -451,7 +460,7 @@ k_row_inv_growth_pair(const CreatureDev* __restrict__ descs, const float2* __res
         }
       }
     });
-    fft_run<N, E, PAIRS, false, true, kTwN, 0, true>(v, lds, t, pr, twl);
+    fft_run<N, E, PAIRS, false, true, kTwN, 0, true, true>(v, lds, t, pr, twl, twr.w);
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const int x = out_pos<N, E, true>(t, i);
