@@ -243,7 +243,8 @@ struct cn_engine {
   std::vector<void*> allocs;
   hipGraphExec_t step_graph[2] = {nullptr, nullptr};
   hipGraphExec_t render_graph[2] = {nullptr, nullptr};  // step + render epilogue
-  hipGraphExec_t multi_graph[2] = {nullptr, nullptr};   // kMultiSteps steps (run_steps)
+  // run_steps graphs of 8, 4 and 2 steps (kRunLen[r]), per ping-pong parity
+  hipGraphExec_t multi_graph[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
   RingParams* gen_rings = nullptr;  // device generator parameters, kMaxKernels entries
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   hipEvent_t join[2] = {nullptr, nullptr};  // cross-stream ordering (cn_iterate_on_stream)
@@ -293,7 +294,7 @@ void dfree(cn_engine* e, std::vector<void*>& owner, void* p, size_t bytes) {
 }
 
 void drop_graphs(cn_engine* e) {
-  for (auto* gs : {e->step_graph, e->render_graph, e->multi_graph})
+  for (auto* gs : {e->step_graph, e->render_graph, e->multi_graph[0], e->multi_graph[1], e->multi_graph[2]})
     for (int j = 0; j < 2; ++j)
       if (gs[j]) {
         (void)hipGraphExecDestroy(gs[j]);
@@ -451,7 +452,7 @@ cn_status capture(cn_engine* e, hipGraphExec_t* out, F&& fn) {
   return CN_OK;
 }
 
-constexpr int kMultiSteps = 8;  // steps per run_steps graph (even)
+constexpr int kRunLen[3] = {8, 4, 2};  // steps per run_steps graph (even: the parity returns)
 
 // One step as a graph launch on `on` (the engine's stream unless a caller's stream is given:
 // then the caller's stream first waits for the engine's queued work and the engine's stream
@@ -1073,32 +1074,37 @@ cn_status cn_run_steps(cn_engine* e, uint64_t n) {
   clear_error();
   if (!e) return set_error(CN_ERR_INVALID_ARG, "null engine");
   CN_HIP(hipSetDevice(e->device));
-  // Runs of kMultiSteps steps replay one graph holding all of them (an even count, so the
-  // ping-pong parity is back where it started): one graph launch per run instead of one per
-  // step, which saves the launch boundary (about 4.5 us per 2048^2 step, 2.3 %).  Both
-  // parities' graphs are captured by the first call, whatever its step count, so that later
-  // calls (a timed run after a short warm-up, say) only launch.
-  for (int par = 0; par < 2; ++par) {
-    if (e->multi_graph[par]) continue;
-    const cn_status st = capture(e, &e->multi_graph[par], [&] {
-      for (int j = 0; j < kMultiSteps; ++j) {
-        const int a = par ^ (j & 1);
-        const hipError_t err = launch_all_steps(e, a, a ^ 1);
-        if (err != hipSuccess) return err;
-      }
-      return hipSuccess;
-    });
-    if (st != CN_OK) return st;
+  // Runs of 8, 4 and 2 steps replay one graph holding all of them (an even count, so the
+  // ping-pong parity is back where it started): n steps take n / 8 launches of the 8-step graph,
+  // then at most one of the 4- and 2-step graphs and one single step, instead of one launch per
+  // step; each launch saved is a launch boundary (about 4.5 us at 2048^2).  Every parity's
+  // graphs are captured by the first call, whatever its step count, so that later calls (a
+  // timed run after a short warm-up, say) only launch.
+  for (int r = 0; r < 3; ++r)
+    for (int par = 0; par < 2; ++par) {
+      if (e->multi_graph[r][par]) continue;
+      const cn_status st = capture(e, &e->multi_graph[r][par], [&] {
+        for (int j = 0; j < kRunLen[r]; ++j) {
+          const int a = par ^ (j & 1);
+          const hipError_t err = launch_all_steps(e, a, a ^ 1);
+          if (err != hipSuccess) return err;
+        }
+        return hipSuccess;
+      });
+      if (st != CN_OK) return st;
+    }
+  for (int r = 0; r < 3; ++r) {
+    const uint64_t len = (uint64_t)kRunLen[r];
+    while (n >= len) {
+      CN_HIP(hipGraphLaunch(e->multi_graph[r][e->cur], e->stream));
+      e->steps += len;
+      e->have_prev = true;
+      e->res.fwd_valid = false;
+      n -= len;
+      if (r > 0) break;  // n < 2 len here
+    }
   }
-  while (n >= (uint64_t)kMultiSteps) {
-    const int cur = e->cur;
-    CN_HIP(hipGraphLaunch(e->multi_graph[cur], e->stream));
-    e->steps += kMultiSteps;
-    e->have_prev = true;
-    e->res.fwd_valid = false;
-    n -= kMultiSteps;
-  }
-  for (uint64_t j = 0; j < n; ++j) {
+  if (n == 1) {
     const cn_status st = enqueue_step(e);
     if (st != CN_OK) return st;
   }

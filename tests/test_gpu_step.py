@@ -186,15 +186,17 @@ def test_multi_step_graph_equals_single_steps(cn, n, fwd_e):
         a.set_plan(n, fwd_e=fwd_e)
         b.set_plan(n, fwd_e=fwd_e)
     a.iterate()                      # odd parity before the multi-step runs
-    a.run_steps(21)
+    a.run_steps(21)                  # 8 + 8 + 4 + 1
     mid = a.download_all_channels()  # eager readback between replays
-    a.run_steps(16)
-    for _ in range(38):
+    a.run_steps(16)                  # 8 + 8
+    a.run_steps(7)                   # 4 + 2 + 1, from even parity
+    a.run_steps(6)                   # 4 + 2, from odd parity
+    for _ in range(51):
         b.iterate()
         if b.info()["steps_done"] == 22:
             assert np.array_equal(b.download_all_channels(), mid)
     assert np.array_equal(a.download_all_channels(), b.download_all_channels())
-    assert a.info()["steps_done"] == 38
+    assert a.info()["steps_done"] == 51
     a.close()
     b.close()
 
