@@ -89,3 +89,48 @@ def test_rollout_survives_each_injected_failure(cn):
 
     n = _walk(cn, lambda: _engine(cn, cr, target), lambda e: cn.lib().cn_rollout_grad(e._h, 6, 2), same)
     assert n >= 10  # state and intermediate slots, two captures, the launch
+
+
+def test_mixed_batch_creation_and_step_survive_each_injected_failure(cn):
+    """A batch of three grid sizes (branch streams and events per size class): every checked call
+    of its creation fails in turn and leaves no engine behind; every checked call of its first
+    step (graph capture with one branch per class) fails in turn and the same batch then steps to
+    the results of a batch that never failed."""
+    from cella_nova_amd._lib import CellaNovaError
+    crs = [cn.baseline_creature(n, num_channels=C, radius=4, seed=40 + n, patch=min(32, n // 2))
+           for n, C in ((16, 2), (32, 3), (64, 2))]
+
+    def make():
+        b = cn.CreatureBatch(crs)
+        for i, cr in enumerate(crs):
+            b.load(i, cr)
+        return b
+
+    ref = make()
+    ref.iterate()
+    want = [ref.creature_download(i) for i in range(len(crs))]
+    ref.close()
+    lib = cn.lib()
+    fails = 0
+    for skip in range(200):
+        lib.cn_debug_fail_hip_at(skip)
+        try:
+            b = cn.CreatureBatch(crs)
+        except CellaNovaError as err:
+            lib.cn_debug_fail_hip_at(-1)
+            assert err.status == CN_ERR_HIP and "injected" in err.message, skip
+            fails += 1
+            continue
+        lib.cn_debug_fail_hip_at(-1)
+        b.close()
+        break
+    else:
+        pytest.fail("batch creation still reached the injection after 200 checked calls")
+    assert fails >= 8  # device, stream, events, descriptor tables, branch streams and events
+
+    def same(b):
+        for i in range(len(crs)):
+            assert np.array_equal(b.creature_download(i), want[i]), i
+
+    n = _walk(cn, make, lambda b: lib.cn_iterate(b._h), same)
+    assert n >= 2
