@@ -204,6 +204,14 @@ struct SizeClass {
   bool col_split = false;
 };
 
+// A contiguous range of one size class's creatures (descriptors d_descs[first .. first+count)):
+// the unit that the step and rollout graphs put on a branch of their own.  Groups share no data,
+// so with more than one group each runs on a branch stream, forked from and joined back into the
+// engine's stream, and the groups' passes overlap (a column pass beside row or stencil passes).
+struct LaunchGroup {
+  int cls = 0, first = 0, count = 0;
+};
+
 // Captured rollout graphs of one (T, checkpoint stride) plan.
 struct Rollout {
   uint32_t T = 0, ckpt = 0;
@@ -248,10 +256,11 @@ struct cn_engine {
   RingParams* gen_rings = nullptr;  // device generator parameters, kMaxKernels entries
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   hipEvent_t join[2] = {nullptr, nullptr};  // cross-stream ordering (cn_iterate_on_stream)
-  // With more than one size class: one branch stream per class and fork/join events, so that
-  // the classes' launches of a step (or of a run of steps) form parallel branches of the graph.
+  // Launch groups (see LaunchGroup) and, with more than one, a branch stream per group and the
+  // fork/join events of the branches.
+  std::vector<LaunchGroup> grp;
   std::vector<hipStream_t> bstream;
-  std::vector<hipEvent_t> bev;  // [class] join events, [classes] the fork event
+  std::vector<hipEvent_t> bev;  // [group] join events, [groups] the fork event
   Rollout ro;               // last captured rollout plan
   RolloutResult res;        // results of the last rollout forward / backward
   // cn_adam_step writes (mu, sigma, inv_s2, w) into the device descriptors: until they are read
@@ -406,62 +415,77 @@ StepLaunch class_launch(cn_engine* e, const SizeClass& k, int sin, int sout) {
   return sl;
 }
 
-// `steps` steps of every size class, the first from state slot `sin` into `sout`, then
-// alternating between the two (ping-pong).  Classes are
-// independent, so with branch streams each class's launches run on a branch of their own (forked
-// from and joined back into the engine's stream): the classes' passes overlap, a column pass of
-// one grid size beside the row or stencil passes of another (config 3's uneven batch: 4.75 ->
-// 4.58 ms per step).  Without branches (one class) the launches go to the engine's stream.
-hipError_t launch_runs(cn_engine* e, int sin, int sout, int steps, bool render = false) {
-  const size_t ncls = e->cls.size();
-  const bool branch = !e->bstream.empty();
-  hipError_t err;
-  if (branch && (err = hipEventRecord(e->bev[ncls], e->stream)) != hipSuccess) return err;
-  for (size_t q = 0; q < ncls; ++q) {
-    const hipStream_t st = branch ? e->bstream[q] : e->stream;
-    if (branch && (err = hipStreamWaitEvent(st, e->bev[ncls], 0)) != hipSuccess) return err;
-    for (int j = 0; j < steps; ++j) {
-      StepLaunch sl = (j & 1) ? class_launch(e, e->cls[q], sout, sin) : class_launch(e, e->cls[q], sin, sout);
-      sl.render = render;
-      if ((err = launch_step(e->cls[q].N, sl, st)) != hipSuccess) return err;
-    }
-    if (branch) {
-      if ((err = hipEventRecord(e->bev[q], st)) != hipSuccess) return err;
-      if ((err = hipStreamWaitEvent(e->stream, e->bev[q], 0)) != hipSuccess) return err;
-    }
+// The launch of group q's creatures: its class's geometry over its range of descriptors.
+StepLaunch group_launch(cn_engine* e, int q, int sin, int sout) {
+  const LaunchGroup& g = e->grp[q];
+  StepLaunch sl = class_launch(e, e->cls[g.cls], sin, sout);
+  sl.descs = e->cls[g.cls].d_descs + g.first;
+  sl.count = g.count;
+  return sl;
+}
+hipStream_t group_stream(const cn_engine* e, int q) { return e->bstream.empty() ? e->stream : e->bstream[q]; }
+int group_n(const cn_engine* e, int q) { return e->cls[e->grp[q].cls].N; }
+
+// Branches: every group's stream waits for the work queued so far on the engine's stream
+// (fork), and the engine's stream waits for every group's work (join).  No-ops with one group.
+hipError_t fork_groups(cn_engine* e) {
+  if (e->bstream.empty()) return hipSuccess;
+  const size_t ng = e->grp.size();
+  hipError_t err = hipEventRecord(e->bev[ng], e->stream);
+  for (size_t q = 0; q < ng && err == hipSuccess; ++q) err = hipStreamWaitEvent(e->bstream[q], e->bev[ng], 0);
+  return err;
+}
+hipError_t join_groups(cn_engine* e) {
+  if (e->bstream.empty()) return hipSuccess;
+  hipError_t err = hipSuccess;
+  for (size_t q = 0; q < e->grp.size() && err == hipSuccess; ++q) {
+    err = hipEventRecord(e->bev[q], e->bstream[q]);
+    if (err == hipSuccess) err = hipStreamWaitEvent(e->stream, e->bev[q], 0);
+  }
+  return err;
+}
+
+// `steps` steps of group q on its stream, the first from state slot `sin` into `sout`, then
+// alternating between the two (ping-pong).
+hipError_t launch_group_runs(cn_engine* e, int q, int sin, int sout, int steps, bool render = false) {
+  for (int j = 0; j < steps; ++j) {
+    StepLaunch sl = (j & 1) ? group_launch(e, q, sout, sin) : group_launch(e, q, sin, sout);
+    sl.render = render;
+    const hipError_t err = launch_step(group_n(e, q), sl, group_stream(e, q));
+    if (err != hipSuccess) return err;
   }
   return hipSuccess;
+}
+
+// `steps` steps of every creature, each group on its branch.
+hipError_t launch_runs(cn_engine* e, int sin, int sout, int steps, bool render = false) {
+  hipError_t err = fork_groups(e);
+  for (size_t q = 0; q < e->grp.size() && err == hipSuccess; ++q)
+    err = launch_group_runs(e, (int)q, sin, sout, steps, render);
+  return err == hipSuccess ? join_groups(e) : err;
 }
 
 hipError_t launch_all_steps(cn_engine* e, int sin, int sout, bool render = false) {
   return launch_runs(e, sin, sout, 1, render);
 }
 
-// A step of every class that keeps its intermediates (u with write_u, A) in slot `islot`;
+// A step of group q that keeps its intermediates (u with write_u, A) in slot `islot`;
 // skip_diffusion stops before the new state.
-hipError_t launch_all_keep(cn_engine* e, int sin, int sout, bool skip_diffusion, int islot) {
-  for (const SizeClass& k : e->cls) {
-    StepLaunch sl = class_launch(e, k, sin, sout);
-    sl.write_u = true;
-    sl.skip_diffusion = skip_diffusion;
-    sl.islot = islot;
-    const hipError_t err = launch_step(k.N, sl, e->stream);
-    if (err != hipSuccess) return err;
-  }
-  return hipSuccess;
+hipError_t launch_group_keep(cn_engine* e, int q, int sin, int sout, bool skip_diffusion, int islot) {
+  StepLaunch sl = group_launch(e, q, sin, sout);
+  sl.write_u = true;
+  sl.skip_diffusion = skip_diffusion;
+  sl.islot = islot;
+  return launch_step(group_n(e, q), sl, group_stream(e, q));
 }
 
-// Adjoint step of state slot `sin` from the intermediates in slot `islot` (recomputed first
-// when `recompute`).
-hipError_t launch_all_adjoint(cn_engine* e, int sin, int gin, int gout, int islot, bool recompute) {
-  for (const SizeClass& k : e->cls) {
-    StepLaunch sl = class_launch(e, k, sin, sin);
-    sl.islot = islot;
-    sl.adj_recompute = recompute;
-    const hipError_t err = launch_adjoint(k.N, sl, gin, gout, e->stream);
-    if (err != hipSuccess) return err;
-  }
-  return hipSuccess;
+// Adjoint step of group q from state slot `sin` and the intermediates in slot `islot`
+// (recomputed first when `recompute`).
+hipError_t launch_group_adjoint(cn_engine* e, int q, int sin, int gin, int gout, int islot, bool recompute) {
+  StepLaunch sl = group_launch(e, q, sin, sin);
+  sl.islot = islot;
+  sl.adj_recompute = recompute;
+  return launch_adjoint(group_n(e, q), sl, gin, gout, group_stream(e, q));
 }
 
 // Captures `fn` (a sequence of launches on e->stream) into an executable graph.
@@ -655,7 +679,8 @@ SlotPlan plan_slots(uint32_t T, uint32_t ck) {
 
 // The steps of the last segment keep their states (segment slots) and intermediates (slot
 // t - tl), so a backward recorded right after it can skip that segment's recompute.
-hipError_t record_forward(cn_engine* e, const SlotPlan& p, uint32_t T, uint32_t ck, int start) {
+// Group q's forward (on its stream).
+hipError_t record_forward(cn_engine* e, int q, const SlotPlan& p, uint32_t T, uint32_t ck, int start) {
   const uint32_t tl = (uint32_t)(p.nck - 1) * ck;  // first step of the last segment
   int in = start;
   for (uint32_t t = 0; t < T; ++t) {
@@ -664,8 +689,8 @@ hipError_t record_forward(cn_engine* e, const SlotPlan& p, uint32_t T, uint32_t 
     else if ((t + 1) % ck == 0) out = p.ckslot((int)((t + 1) / ck), start);
     else if (t + 1 > tl) out = p.segslot((int)(t + 1 - tl));
     else out = p.x0 + (int)(t % 2);
-    const hipError_t err =
-        t >= tl ? launch_all_keep(e, in, out, false, (int)(t - tl)) : launch_all_steps(e, in, out);
+    const hipError_t err = t >= tl ? launch_group_keep(e, q, in, out, false, (int)(t - tl))
+                                   : launch_group_runs(e, q, in, out, 1);
     if (err != hipSuccess) return err;
     in = out;
   }
@@ -674,11 +699,11 @@ hipError_t record_forward(cn_engine* e, const SlotPlan& p, uint32_t T, uint32_t 
 
 // Backward from gradient slot 0 (dL/dfinal); returns the slot holding dL/dstate0.  last_kept:
 // the last segment's states and intermediates are still those record_forward left.
-hipError_t record_backward(cn_engine* e, const SlotPlan& p, uint32_t T, uint32_t ck, int start, bool last_kept,
-                           int* gslot) {
+hipError_t record_backward(cn_engine* e, int q, const SlotPlan& p, uint32_t T, uint32_t ck, int start,
+                           bool last_kept, int* gslot) {
   hipError_t err;
-  for (const SizeClass& k : e->cls)
-    if ((err = launch_zero_gparam(k.d_descs, (int)k.members.size(), e->stream)) != hipSuccess) return err;
+  const StepLaunch gl = group_launch(e, q, 0, 1);
+  if ((err = launch_zero_gparam(gl.descs, gl.count, group_stream(e, q))) != hipSuccess) return err;
   int g = 0;
   for (int j = p.nck - 1; j >= 0; --j) {
     const uint32_t t0 = (uint32_t)j * ck, t1 = std::min(T, t0 + ck);
@@ -688,12 +713,12 @@ hipError_t record_backward(cn_engine* e, const SlotPlan& p, uint32_t T, uint32_t
     // next checkpoint or the final state, both stored).
     for (uint32_t t = t0; t < t1 && !(last_kept && j == p.nck - 1); ++t) {
       const bool last = t + 1 == t1;
-      if ((err = launch_all_keep(e, state_slot(t), last ? state_slot(t) : state_slot(t + 1), last, (int)(t - t0))) !=
-          hipSuccess)
+      if ((err = launch_group_keep(e, q, state_slot(t), last ? state_slot(t) : state_slot(t + 1), last,
+                                   (int)(t - t0))) != hipSuccess)
         return err;
     }
     for (uint32_t t = t1; t-- > t0;) {
-      if ((err = launch_all_adjoint(e, state_slot(t), g, g ^ 1, (int)(t - t0), false)) != hipSuccess) return err;
+      if ((err = launch_group_adjoint(e, q, state_slot(t), g, g ^ 1, (int)(t - t0), false)) != hipSuccess) return err;
       g ^= 1;
     }
   }
@@ -717,14 +742,23 @@ cn_status prepare_rollout(cn_engine* e, uint32_t T, uint32_t ck) {
   if (e->ro.fwd) (void)hipGraphExecDestroy(e->ro.fwd);
   e->ro = Rollout{};
   // Graphs always start from slot 0: the engine moves its current state there first.
-  if ((st = capture(e, &e->ro.fwd, [&] { return record_forward(e, p, T, ck, 0); })) != CN_OK) return st;
+  // Each launch group records its whole forward (and loss and backward) on its own branch.
+  st = capture(e, &e->ro.fwd, [&] {
+    hipError_t err = fork_groups(e);
+    for (size_t q = 0; q < e->grp.size() && err == hipSuccess; ++q) err = record_forward(e, (int)q, p, T, ck, 0);
+    return err == hipSuccess ? join_groups(e) : err;
+  });
+  if (st != CN_OK) return st;
   int gslot = 0;
   st = capture(e, &e->ro.graph, [&] {
-    hipError_t err = record_forward(e, p, T, ck, 0);
-    if (err != hipSuccess) return err;
-    for (const SizeClass& k : e->cls)
-      if ((err = launch_mse_grad(k.d_descs, (int)k.members.size(), p.fin, 0, e->stream)) != hipSuccess) return err;
-    return record_backward(e, p, T, ck, 0, true, &gslot);
+    hipError_t err = fork_groups(e);
+    for (size_t q = 0; q < e->grp.size() && err == hipSuccess; ++q) {
+      if ((err = record_forward(e, (int)q, p, T, ck, 0)) != hipSuccess) break;
+      const StepLaunch gl = group_launch(e, (int)q, 0, 1);
+      if ((err = launch_mse_grad(gl.descs, gl.count, p.fin, 0, group_stream(e, (int)q))) != hipSuccess) break;
+      err = record_backward(e, (int)q, p, T, ck, 0, true, &gslot);
+    }
+    return err == hipSuccess ? join_groups(e) : err;
   });
   if (st != CN_OK) return st;
   e->ro.T = T;
@@ -846,9 +880,19 @@ cn_status create_engine(const cn_engine_desc* descs, uint32_t count, int device,
     if ((st = dalloc(e, e->allocs, &p, k.members.size() * sizeof(CreatureDev))) != CN_OK) return fail(st);
     k.d_descs = (CreatureDev*)p;
   }
-  if (e->cls.size() > 1) {  // branch streams (default priority) and their fork/join events
-    e->bstream.assign(e->cls.size(), nullptr);
-    e->bev.assign(e->cls.size() + 1, nullptr);
+  // Launch groups: every size class of several creatures is split in two halves, each a branch
+  // of the graphs, so that one half's column pass overlaps the other's row and stencil passes
+  // (and other classes' passes).  Measured: 32 x 512^2 C5K25 search 12,500 -> 13,600
+  // grad-steps/s (four quarters: 12,900); the uneven 256-creature batch 4.77 ms per step with
+  // the classes in sequence, 4.59 with one branch per class, 4.45 with two.
+  for (size_t j = 0; j < e->cls.size(); ++j) {
+    const int m = (int)e->cls[j].members.size();
+    const int parts = m >= 2 ? 2 : 1;
+    for (int q = 0; q < parts; ++q) e->grp.push_back(LaunchGroup{(int)j, m * q / parts, m * (q + 1) / parts - m * q / parts});
+  }
+  if (e->grp.size() > 1) {  // branch streams (default priority) and their fork/join events
+    e->bstream.assign(e->grp.size(), nullptr);
+    e->bev.assign(e->grp.size() + 1, nullptr);
     for (auto& q : e->bstream) {
       if (inject_failure()) return fail(set_error(CN_ERR_HIP, "hipStreamCreate: injected failure"));
       if (hip_failed(hipStreamCreateWithFlags(&q, hipStreamNonBlocking), "hipStreamCreate", &st)) return fail(st);
@@ -1576,7 +1620,9 @@ cn_status cn_rollout_backward(cn_engine* e, const float* const* d_grad_final, ui
   const SlotPlan p = plan_slots(e->res.T, e->res.ckpt);
   int gslot = 0;
   // Other calls may have used intermediate slot 0 since the forward: recompute every segment.
-  CN_HIP(record_backward(e, p, e->res.T, e->res.ckpt, 0, false, &gslot));
+  CN_HIP(fork_groups(e));
+  for (size_t q = 0; q < e->grp.size(); ++q) CN_HIP(record_backward(e, (int)q, p, e->res.T, e->res.ckpt, 0, false, &gslot));
+  CN_HIP(join_groups(e));
   CN_HIP(hipStreamSynchronize(e->stream));
   e->res.grad_slot = gslot;
   e->res.grads_valid = true;
