@@ -170,7 +170,7 @@ k_growth_adj_row(const CreatureDev* __restrict__ descs, const float2* __restrict
 
 // ---------------------------------------------------------------------------------------
 // Column adjoint: for one source channel and column tile, sum over the kernels reading that
-// source of FFT_col(rowspec(gu_k)) * conj(K^_k), then one inverse column FFT into spec[c].
+// source of FFT_col(rowspec(gu_k)) * conj(K^_k), then one inverse column FFT into adjspec[c].
 // ---------------------------------------------------------------------------------------
 template <int N, int TC, int EC>
 __global__ void __launch_bounds__((TC * ColCfg<N, EC>::T)) __attribute__((amdgpu_waves_per_eu(4)))
@@ -259,13 +259,13 @@ k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) 
   }
   fft_run<L, E, TC, true, true, kTwN, 0, true>(acc, lds, t, lane, twl + TF::size);
   // Result layout of the column passes (res_index), as the forward column pass writes uhat.
-  gptr<float2> out = c64(d.spec) + (size_t)c * N * HALF;
+  gptr<float2> out = c64(d.adjspec) + (size_t)c * N * HALF;
 #pragma unroll
   for (int i = 0; i < E; ++i) out[res_index(N, out_pos<L, E, true>(t, i), x)] = acc[i];
 }
 
 // ---------------------------------------------------------------------------------------
-// Row C2R of spec[c] (the convolution part of dL/ds) plus the diffusion part -> grad slot.
+// Row C2R of adjspec[c] (the convolution part of dL/ds) plus the diffusion part -> grad slot.
 // ---------------------------------------------------------------------------------------
 template <int N, int ER>
 __global__ void __launch_bounds__((RowCfg<N, ER>::THREADS))
@@ -288,7 +288,7 @@ k_row_inv_add(const CreatureDev* __restrict__ descs, int gout, const float2* __r
   }
   __syncthreads();
   float2 v[E];
-  c2r_start_ct<N, E, true>((const float2*)d.spec + (size_t)c * N * L, y, t, twl + TI::size, v);
+  c2r_start_ct<N, E, true>((const float2*)d.adjspec + (size_t)c * N * L, y, t, twl + TI::size, v);
   fft_run<L, E, ROWS, false, true, kTwN, 0, true>(v, lds, t, lane, twl);
   const size_t row = ((size_t)c * N + y) * N;
   gptr<const float2> dd = c64((const void*)(d.dsd + row));

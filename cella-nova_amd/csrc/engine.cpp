@@ -8,6 +8,7 @@
 // The single-creature C ABI of include/cellanova.h is the batch API with creature 0.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -259,6 +260,10 @@ struct cn_engine {
   // Launch groups (see LaunchGroup) and, with more than one, a branch stream per group and the
   // fork/join events of the branches.
   std::vector<LaunchGroup> grp;
+  // Backward pipelines (allocated with the first rollout): per group a recompute stream beside
+  // the group's stream and the events that order the two (see record_backward).
+  std::vector<hipStream_t> rstream;
+  std::vector<std::array<hipEvent_t, 5>> pev;  // [fork, recompute done, adjoint done (2 parities), join]
   std::vector<hipStream_t> bstream;
   std::vector<hipEvent_t> bev;  // [group] join events, [groups] the fork event
   Rollout ro;               // last captured rollout plan
@@ -329,6 +334,13 @@ void release(cn_engine* e) {
     if (v) (void)hipEventDestroy(v);
   for (auto& q : e->bstream)
     if (q) (void)hipStreamSynchronize(q);
+  for (auto& q : e->rstream)
+    if (q) (void)hipStreamSynchronize(q);
+  for (auto& a : e->pev)
+    for (auto& v : a)
+      if (v) (void)hipEventDestroy(v);
+  for (auto& q : e->rstream)
+    if (q) (void)hipStreamDestroy(q);
   for (auto& v : e->bev)
     if (v) (void)hipEventDestroy(v);
   for (auto& q : e->bstream)
@@ -602,6 +614,7 @@ cn_status ensure_adjoint(cn_engine* e) {
     if ((st = need(d.gaff, C * P * 4)) != CN_OK) return st;
     if ((st = need(d.dsd, C * P * 4)) != CN_OK) return st;
     if ((st = need(d.specg, K * c.N * H * 8)) != CN_OK) return st;
+    if ((st = need(d.adjspec, C * c.N * H * 8)) != CN_OK) return st;
     if ((st = need(d.partial, (size_t)row_blocks((int)c.N) * K * 3 * 4)) != CN_OK) return st;
     if ((st = need(d.gparam, 6 * K * 4)) != CN_OK) return st;
     if ((st = need(d.target, C * P * 4)) != CN_OK) return st;
@@ -661,10 +674,16 @@ cn_status ensure_islots(cn_engine* e, int want) {
 // `start` (0 or 1, left unmodified):
 //   ck[0] = start; ck[j] (1 <= j < nck) = slot 2 + j - 1; forward scratch X0, X1;
 //   recompute buffers seg[i] (1 <= i < ck); final state.
+// State slots of a rollout: 0/1 ping-pong (the start is slot 0), checkpoints 2.. for segments
+// 1.., two scratch slots x0, x0 + 1 for the forward, and two sets of ck - 1 segment slots (the
+// states inside a segment) so that one segment can be recomputed while the adjoint still reads
+// the states of the next one.  Segment j uses set par(j) = (nck - 1 - j) mod 2: the forward
+// keeps the last segment's states in set 0.  fin holds the final state.
 struct SlotPlan {
-  int nck, x0, seg0, fin, total;
+  int nck, x0, seg0, fin, seg1, total;
   int ckslot(int j, int start) const { return j == 0 ? start : 2 + j - 1; }
-  int segslot(int i) const { return seg0 + i - 1; }
+  int segslot(int i, int par = 0) const { return (par ? seg1 : seg0) + i - 1; }
+  int par(int j) const { return (nck - 1 - j) & 1; }
 };
 SlotPlan plan_slots(uint32_t T, uint32_t ck) {
   if (ck == 0) ck = 1;  // callers validate T, ck > 0; never divide by zero
@@ -673,7 +692,8 @@ SlotPlan plan_slots(uint32_t T, uint32_t ck) {
   p.x0 = 2 + p.nck - 1;
   p.seg0 = p.x0 + 2;
   p.fin = p.seg0 + (int)ck - 1;
-  p.total = p.fin + 1;
+  p.seg1 = p.fin + 1;
+  p.total = p.seg1 + (int)ck - 1;
   return p;
 }
 
@@ -697,33 +717,90 @@ hipError_t record_forward(cn_engine* e, int q, const SlotPlan& p, uint32_t T, ui
   return hipSuccess;
 }
 
-// Backward from gradient slot 0 (dL/dfinal); returns the slot holding dL/dstate0.  last_kept:
-// the last segment's states and intermediates are still those record_forward left.
+// Backward of group q from gradient slot 0 (dL/dfinal); returns the slot holding dL/dstate0.
+// last_kept: the last segment's states and intermediates are still those record_forward left.
+//
+// Segment j's recompute (from its checkpoint, keeping each step's states and intermediates) only
+// needs the checkpoint, so it runs on the group's recompute stream while the adjoint, on the
+// group's stream, still walks back through segment j + 1: segments alternate between two sets
+// of segment-state and intermediate slots (par(j)), the adjoint of segment j waits for its
+// recompute (event 1), and the recompute of segment j waits for the adjoint of segment j + 2,
+// the last reader of the same set (events 2 + par, recorded after each segment's adjoint).  The
+// adjoint steps themselves stay in order on one stream, so gradients are bitwise those of the
+// sequential schedule.
+//
+// Only engines with a single launch group pipeline their backward (the recompute stream forks
+// from the engine's own stream).  With several groups the recompute stream would fork from a
+// group's branch stream, itself forked from the engine's stream; the one run of that graph (the
+// 32-creature search) ended the process with a segmentation fault in the host runtime whose
+// cause is not known, so those engines recompute on the group's stream, in sequence.
 hipError_t record_backward(cn_engine* e, int q, const SlotPlan& p, uint32_t T, uint32_t ck, int start,
                            bool last_kept, int* gslot) {
   hipError_t err;
+  const bool pipe = e->grp.size() == 1;
+  const hipStream_t as = group_stream(e, q), rs = pipe ? e->rstream[q] : as;
+  const auto& ev = e->pev[q];
   const StepLaunch gl = group_launch(e, q, 0, 1);
-  if ((err = launch_zero_gparam(gl.descs, gl.count, group_stream(e, q))) != hipSuccess) return err;
+  if ((err = launch_zero_gparam(gl.descs, gl.count, as)) != hipSuccess) return err;
+  if (pipe) {  // fork the recompute stream
+    if ((err = hipEventRecord(ev[0], as)) != hipSuccess) return err;
+    if ((err = hipStreamWaitEvent(rs, ev[0], 0)) != hipSuccess) return err;
+  }
+  hipError_t jerr = hipSuccess;
+  auto join = [&] {  // the recompute stream back into the group's stream (also after an error)
+    if (!pipe) return hipSuccess;
+    hipError_t e1 = hipEventRecord(ev[4], rs);
+    if (e1 == hipSuccess) e1 = hipStreamWaitEvent(as, ev[4], 0);
+    return e1;
+  };
   int g = 0;
   for (int j = p.nck - 1; j >= 0; --j) {
     const uint32_t t0 = (uint32_t)j * ck, t1 = std::min(T, t0 + ck);
-    auto state_slot = [&](uint32_t t) { return t == t0 ? p.ckslot(j, start) : p.segslot((int)(t - t0)); };
+    const int par = p.par(j), ib = par * (int)ck;
+    auto state_slot = [&](uint32_t t) { return t == t0 ? p.ckslot(j, start) : p.segslot((int)(t - t0), par); };
     // Recompute the segment from its checkpoint, keeping each step's intermediates (u, A) in
-    // intermediate slot t - t0; the last step stops before its diffusion (its output is the
+    // intermediate slot ib + t - t0; the last step stops before its diffusion (its output is the
     // next checkpoint or the final state, both stored).
-    for (uint32_t t = t0; t < t1 && !(last_kept && j == p.nck - 1); ++t) {
-      const bool last = t + 1 == t1;
-      if ((err = launch_group_keep(e, q, state_slot(t), last ? state_slot(t) : state_slot(t + 1), last,
-                                   (int)(t - t0))) != hipSuccess)
-        return err;
+    if (!(last_kept && j == p.nck - 1)) {
+      if (pipe && j + 2 < p.nck && (err = hipStreamWaitEvent(rs, ev[2 + par], 0)) != hipSuccess) break;
+      for (uint32_t t = t0; t < t1 && err == hipSuccess; ++t) {
+        const bool last = t + 1 == t1;
+        StepLaunch sl = group_launch(e, q, state_slot(t), last ? state_slot(t) : state_slot(t + 1));
+        sl.write_u = true;
+        sl.skip_diffusion = last;
+        sl.islot = ib + (int)(t - t0);
+        err = launch_step(group_n(e, q), sl, rs);
+      }
+      if (err != hipSuccess) break;
+      if (pipe && ((err = hipEventRecord(ev[1], rs)) != hipSuccess || (err = hipStreamWaitEvent(as, ev[1], 0)) != hipSuccess))
+        break;
     }
-    for (uint32_t t = t1; t-- > t0;) {
-      if ((err = launch_group_adjoint(e, q, state_slot(t), g, g ^ 1, (int)(t - t0), false)) != hipSuccess) return err;
+    for (uint32_t t = t1; t-- > t0 && err == hipSuccess;) {
+      err = launch_group_adjoint(e, q, state_slot(t), g, g ^ 1, ib + (int)(t - t0), false);
       g ^= 1;
     }
+    if (err != hipSuccess) break;
+    if (pipe && (err = hipEventRecord(ev[2 + par], as)) != hipSuccess) break;
   }
+  jerr = join();
+  if (err != hipSuccess) return err;
+  if (jerr != hipSuccess) return jerr;
   *gslot = g;
   return hipSuccess;
+}
+
+// The recompute streams and pipeline events of every group (first rollout; kept on a failure
+// part-way, so that a retry creates only what is missing).
+cn_status ensure_pipeline(cn_engine* e) {
+  const size_t ng = e->grp.size();
+  if (e->rstream.size() != ng) e->rstream.resize(ng, nullptr);
+  if (e->pev.size() != ng) e->pev.resize(ng, {nullptr, nullptr, nullptr, nullptr, nullptr});
+  for (size_t q = 0; q < ng; ++q) {
+    if (!e->rstream[q]) CN_HIP(hipStreamCreateWithFlags(&e->rstream[q], hipStreamNonBlocking));
+    for (auto& v : e->pev[q])
+      if (!v) CN_HIP(hipEventCreateWithFlags(&v, hipEventDisableTiming));
+  }
+  return CN_OK;
 }
 
 cn_status prepare_rollout(cn_engine* e, uint32_t T, uint32_t ck) {
@@ -736,7 +813,8 @@ cn_status prepare_rollout(cn_engine* e, uint32_t T, uint32_t ck) {
   if ((st = ensure_adjoint(e)) != CN_OK) return st;
   const SlotPlan p = plan_slots(T, ck);
   if ((st = ensure_slots(e, p.total)) != CN_OK) return st;
-  if ((st = ensure_islots(e, (int)ck)) != CN_OK) return st;
+  if ((st = ensure_islots(e, 2 * (int)ck)) != CN_OK) return st;  // two sets (record_backward)
+  if ((st = ensure_pipeline(e)) != CN_OK) return st;
   if (e->ro.T == T && e->ro.ckpt == ck && e->ro.graph && e->ro.fwd) return CN_OK;
   if (e->ro.graph) (void)hipGraphExecDestroy(e->ro.graph);
   if (e->ro.fwd) (void)hipGraphExecDestroy(e->ro.fwd);

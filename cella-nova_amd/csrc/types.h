@@ -4,8 +4,7 @@
 //   state   [nslots][C][N][N] f32  world-state slots: 0/1 ping-pong current state (the
 //                                  reference's channel/new_channel), the rest are rollout
 //                                  checkpoints and recompute buffers.  Kernels take slot indices.
-//   spec    [C][N][N/2]  c64   row-R2C spectra of a state (row-major); the adjoint's column pass
-//                              reuses it for its output, laid out like uhat (res_index)
+//   spec    [C][N][N/2]  c64   row-R2C spectra of a state (row-major)
 //   khalf   [K][N][N/2]  c64   Hermitian part of each kernel spectrum, pre-scaled by 1/N^2,
 //                              columns kx = 0 .. N/2-1
 //   knyq    [K][N]       c64   the kx = N/2 (Nyquist) column of the same spectra
@@ -26,6 +25,8 @@
 //   gaff    [C][N][N]    f32   dL/d(pre-exp affinity)
 //   dsd     [C][N][N]    f32   diffusion part of dL/dstate
 //   specg   [K][N][N/2]  c64   row spectra of dL/du_k
+//   adjspec [C][N][N/2]  c64   per-source sums of the column adjoint (res_index layout), input of
+//                              the inverse-row adjoint
 //   partial [nblk][K][3] f32   per-block sums for dL/dmu, dL/dsigma, dL/dw
 //   gparam  [2][3][K]    f32   accumulated (mu, sigma, w) gradients and their Kahan terms
 //   target  [C][N][N]    f32   loss target
@@ -78,6 +79,8 @@ struct CreatureDev {
   float* gaff;
   float* dsd;
   void* specg;         // float2*
+  void* adjspec;       // float2*: the adjoint column pass's output (its own plane, so that a
+                       // segment's recompute may run beside another segment's adjoint)
   float* partial;
   float* gparam;
   float* target;
