@@ -283,7 +283,7 @@ cn_status cn_write_wav(const char* path, const int16_t* samples, size_t count, u
   clear_error();
   if (!path || (!samples && count)) return set_error(CN_ERR_INVALID_ARG, "null argument");
   if (channels == 0 || sample_rate == 0) return set_error(CN_ERR_INVALID_ARG, "channels and sample rate must be positive");
-  if (count * 2 > 0xFFFFFFFFull - 36) return set_error(CN_ERR_INVALID_ARG, "%zu samples exceed a WAV file", count);
+  if (count > (0xFFFFFFFFull - 36) / 2) return set_error(CN_ERR_INVALID_ARG, "%zu samples exceed a WAV file", count);
   FILE* f = std::fopen(path, "wb");
   if (!f) return set_error(CN_ERR_IO, "cannot create %s", path);
   // RIFF/WAVE PCM header (audio.rs:412-441), little-endian fields.
