@@ -27,16 +27,24 @@ HBM_PEAK_GBS = 8000.0
 KERNELS = ["row_fwd_r2c", "col_fused", "row_inv_growth", "diffusion"]
 
 
-def algorithmic_bytes_per_cell(C, K, compact=0):
+def algorithmic_bytes_per_cell(C, K, compact=0, seg_table=None):
     """Bytes each kernel must move per grid site (all channels), by its inputs and outputs:
     row_fwd reads C f32 planes, writes C half spectra (N/2 complex per row = 4 B/site);
     col_fused reads C half spectra + K kernel spectra, writes K half spectra; a kernel spectrum
     is a complex half spectrum (4 B/site) or, for the `compact` kernels whose spectrum is real
-    and even in ky, a real quarter (1 B/site);
+    and even in ky, a real quarter (1 B/site); with the segmented column pass it is the kernel's
+    segment table instead (`seg_table` B/site per kernel: (M/2+1)(N/2+1) f32 over N^2 sites);
     row_inv_growth reads K half spectra, writes C affinity planes;
     diffusion reads C affinity + C state planes, writes C state planes."""
-    return {"row_fwd_r2c": 8 * C, "col_fused": 4 * C + 4 * K + 4 * (K - compact) + compact,
+    kern = K * seg_table if seg_table is not None else 4 * (K - compact) + compact
+    return {"row_fwd_r2c": 8 * C, "col_fused": 4 * C + 4 * K + kern,
             "row_inv_growth": 4 * K + 4 * C, "diffusion": 12 * C}
+
+
+def seg_table_bytes_per_cell(info):
+    """Segment-table bytes per grid site per kernel, or None without the segmented pass."""
+    m, n = info.get("segment_length", 0), info["n"]
+    return (m // 2 + 1) * (n // 2 + 1) * 4 / (n * n) if info.get("segments", 0) and m else None
 
 
 def _cpu_model():
@@ -361,7 +369,7 @@ def main():
                     help="batched-grad: iterations of the PyTorch reference path (0: skip)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--plan", default="",
-                    help="fixed launch plan fwd_e,inv_e,col_e,col_tc,diff_th[,col_split] (no autotune)")
+                    help="fixed launch plan fwd_e,inv_e,col_e,col_tc,diff_th[,col_split[,col_full]] (no autotune)")
     ap.add_argument("--plans", default="",
                     help="batched workloads: JSON list of launch plans (as printed under \"plans\") instead "
                          "of tuning")
@@ -431,9 +439,10 @@ def main():
     # Per-kernel average launch durations (HIP events on the engine stream), same number of steps.
     prof_steps = min(args.steps, 200)
     kms = eng.profile_kernels(prof_steps)
-    compact = eng.info()["compact_kernels"]
+    info = eng.info()
+    compact = info["compact_kernels"]
     eng.close()
-    bpc = algorithmic_bytes_per_cell(C, K, compact)
+    bpc = algorithmic_bytes_per_cell(C, K, compact, seg_table_bytes_per_cell(info))
     per_kernel = {}
     for name, ms in zip(KERNELS, kms):
         b = bpc[name] * n * n
@@ -451,6 +460,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"DiffusionLenia {n}x{n} C{C}K{K} R={R} periodic, temp 1, seed 0",
                    "grid": n, "channels": C, "kernels": K, "radius": R, "compact_kernels": compact,
+                   "column_segments": info["segments"], "segment_radius": info["segment_radius"],
                    "plan": plan},
         "roofline": roof,
         "kernels": per_kernel,

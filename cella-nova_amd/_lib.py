@@ -39,7 +39,8 @@ class EngineDesc(ctypes.Structure):
 
 class EngineInfo(ctypes.Structure):
     _fields_ = [("n", c_uint32), ("num_channels", c_uint32), ("num_kernels", c_uint32),
-                ("steps_done", c_uint64), ("device_bytes", c_uint64), ("compact_kernels", c_uint32)]
+                ("steps_done", c_uint64), ("device_bytes", c_uint64), ("compact_kernels", c_uint32),
+                ("segments", c_uint32), ("segment_radius", c_uint32), ("segment_length", c_uint32)]
 
 
 class Config(ctypes.Structure):
@@ -67,7 +68,7 @@ class AdamParams(ctypes.Structure):
 
 class Plan(ctypes.Structure):
     _fields_ = [("n", c_uint32), ("fwd_e", c_uint32), ("inv_e", c_uint32), ("col_e", c_uint32),
-                ("col_tc", c_uint32), ("diff_th", c_uint32), ("col_split", c_uint32)]
+                ("col_tc", c_uint32), ("diff_th", c_uint32), ("col_split", c_uint32), ("col_full", c_uint32)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/cellanova.h

@@ -52,4 +52,11 @@ __device__ __forceinline__ int xcd_tile(int bx) {
   return (NTILES % 8 == 0) ? (bx % 8) * (NTILES / 8) + bx / 8 : bx;
 }
 
+// Segmented column pass: columns per block (16 complex values: one 128-byte line per row),
+// the largest column-profile support radius it takes, and the segment length per grid size
+// (0: full-length column transforms only).
+constexpr int kSegTC = 16;
+constexpr int kSegRcap = 64;
+constexpr int seg_m(int N) { return N == 2048 ? 256 : 0; }
+
 }  // namespace cn

@@ -192,6 +192,9 @@ struct Creature {
   bool adj = false;        // adjoint buffers allocated
   int islots = 1;          // intermediate slots of A (and of u once allocated)
   bool has_target = false;
+  // Per kernel: support radius of its column profiles for the segmented column pass (-1: the
+  // kernel has none within kSegRcap, or its spectrum is not real and even).
+  std::vector<int> seg_r;
   size_t plane() const { return (size_t)N * N; }
   size_t slot_elems() const { return (size_t)C * N * N; }
 };
@@ -203,6 +206,7 @@ struct SizeClass {
   int max_c = 0, max_k = 0;
   int fwd_e = 0, inv_e = 0, col_e = 0, col_tc = 0, diff_th = 0;  // launch geometry (0: default)
   bool col_split = false;
+  bool col_full = false;  // full-length column transforms even for creatures with segments
 };
 
 // A contiguous range of one size class's creatures (descriptors d_descs[first .. first+count)):
@@ -409,6 +413,107 @@ cn_status commit_desc(cn_engine* e, int i, const CreatureDev& before, bool* rest
   return st;
 }
 
+// Column-profile support radii and segment tables of kernels `ks` of creature i (after their
+// spectra changed), then the creature's segment geometry: seg_g segments of M = seg_m(N) rows
+// overlapping by 2 seg_r, seg_r the largest radius, when every kernel has one (else seg_g = 0,
+// full-length column transforms).  Kernels keep radius -1 when anything fails.  Launch graphs
+// bake the segment count and the choice of column pass in, so a change drops them.
+cn_status build_segments(cn_engine* e, int i, const std::vector<int>& ks) {
+  Creature& c = e->cr[i];
+  const int N = (int)c.N, M = seg_m(N), W = N / 2 + 1, RC = kSegRcap;
+  std::vector<int> todo;
+  for (int k : ks) {
+    c.seg_r[k] = -1;
+    if (M > 0 && c.dev.ksym[k]) todo.push_back(k);
+  }
+  cn_status st = CN_OK;
+  auto ok = [&](hipError_t he, const char* what) {
+    if (st != CN_OK) return false;
+    if (inject_failure()) {
+      st = set_error(CN_ERR_HIP, "%s: injected failure", what);
+      return false;
+    }
+    return !hip_failed(he, what, &st);
+  };
+  if (!todo.empty()) {
+    const int nk = (int)todo.size();
+    std::vector<void*> owner;
+    std::vector<std::pair<void*, size_t>> tmp;
+    auto talloc = [&](size_t bytes) -> void* {
+      void* q = nullptr;
+      if (st == CN_OK && (st = dalloc(e, owner, &q, bytes)) == CN_OK) tmp.emplace_back(q, bytes);
+      return q;
+    };
+    const size_t sb = (size_t)nk * (RC + 2) * 8;
+    double* hbuf = (double*)talloc((size_t)nk * W * (RC + 1) * 8);
+    double* etail = (double*)talloc((size_t)nk * W * (RC + 2) * 8);
+    double* esum = (double*)talloc(sb);
+    int* dks = (int*)talloc((size_t)nk * 4);
+    int* drs = (int*)talloc((size_t)nk * 4);
+    std::vector<double> es((size_t)nk * (RC + 2));
+    const CreatureDev* d = e->cls[c.cls].d_descs + c.idx;
+    if (st == CN_OK && ok(hipMemcpyAsync(dks, todo.data(), (size_t)nk * 4, hipMemcpyHostToDevice, e->stream),
+                          "hipMemcpyAsync") &&
+        ok(launch_seg_profile(N, d, dks, nk, hbuf, etail, e->stream), "segment profiles") &&
+        ok(launch_seg_tail_sum(N, etail, nk, esum, e->stream), "segment tail sums") &&
+        ok(hipMemcpyAsync(es.data(), esum, sb, hipMemcpyDeviceToHost, e->stream), "hipMemcpyAsync") &&
+        ok(hipStreamSynchronize(e->stream), "hipStreamSynchronize")) {
+      // Support: the smallest radius whose tail energy is within (2^-22)^2 of the total.
+      const double tau2 = std::ldexp(1.0, -44);
+      std::vector<int> rad(nk, -1);
+      for (int kk = 0; kk < nk; ++kk) {
+        const double* ek = es.data() + (size_t)kk * (RC + 2);
+        for (int r = 0; r <= RC && rad[kk] < 0; ++r)
+          if (ek[r + 1] <= tau2 * ek[0]) rad[kk] = r;
+      }
+      bool any = false;
+      for (int r : rad) any = any || r >= 0;
+      if (any && ok(hipMemcpyAsync(drs, rad.data(), (size_t)nk * 4, hipMemcpyHostToDevice, e->stream),
+                    "hipMemcpyAsync") &&
+          ok(launch_seg_table(N, hbuf, dks, drs, nk, const_cast<float*>(c.dev.kseg), e->stream), "segment tables") &&
+          ok(hipStreamSynchronize(e->stream), "hipStreamSynchronize")) {
+        for (int kk = 0; kk < nk; ++kk) c.seg_r[todo[kk]] = rad[kk];
+      }
+    }
+    (void)hipStreamSynchronize(e->stream);
+    for (auto& q : tmp) dfree(e, owner, q.first, q.second);
+  }
+  int R = -1;
+  bool all = M > 0;
+  for (int r : c.seg_r) {
+    all = all && r >= 0;
+    R = std::max(R, r);
+  }
+  const int g = all ? (N + (M - 2 * R) - 1) / (M - 2 * R) : 0, r = all ? R : 0;
+  if (g != c.dev.seg_g || r != c.dev.seg_r) {
+    const CreatureDev before = c.dev;
+    c.dev.seg_g = g;
+    c.dev.seg_r = r;
+    drop_graphs(e);
+    bool restored = false;
+    const cn_status cs = commit_desc(e, i, before, &restored);
+    if (st == CN_OK) st = cs;
+  }
+  return st;
+}
+
+// Segmented column pass of creatures members[first .. first+count) of a class: the largest
+// seg_g (0: none has segments, or the class's plan asks for full-length column transforms)
+// and whether some of them have none (the full-length pass then runs for those).
+void set_seg_grid(const cn_engine* e, const SizeClass& k, int first, int count, StepLaunch& sl) {
+  sl.seg_g = 0;
+  sl.seg_mixed = false;
+  if (k.col_full) return;
+  for (int j = first; j < first + count; ++j) {
+    const int cg = e->cr[k.members[j]].dev.seg_g;
+    if (cg > 0)
+      sl.seg_g = std::max(sl.seg_g, cg);
+    else
+      sl.seg_mixed = true;
+  }
+  if (sl.seg_g == 0) sl.seg_mixed = false;
+}
+
 StepLaunch class_launch(cn_engine* e, const SizeClass& k, int sin, int sout) {
   StepLaunch sl;
   sl.descs = k.d_descs;
@@ -424,6 +529,7 @@ StepLaunch class_launch(cn_engine* e, const SizeClass& k, int sin, int sout) {
   sl.col_tc = k.col_tc;
   sl.diff_th = k.diff_th;
   sl.col_split = k.col_split;
+  set_seg_grid(e, k, 0, (int)k.members.size(), sl);
   return sl;
 }
 
@@ -433,6 +539,7 @@ StepLaunch group_launch(cn_engine* e, int q, int sin, int sout) {
   StepLaunch sl = class_launch(e, e->cls[g.cls], sin, sout);
   sl.descs = e->cls[g.cls].d_descs + g.first;
   sl.count = g.count;
+  set_seg_grid(e, e->cls[g.cls], g.first, g.count, sl);
   return sl;
 }
 hipStream_t group_stream(const cn_engine* e, int q) { return e->bstream.empty() ? e->stream : e->bstream[q]; }
@@ -945,6 +1052,11 @@ cn_status create_engine(const cn_engine_desc* descs, uint32_t count, int device,
     d.knyq = p;
     if ((st = dalloc(e, c.allocs, &p, K * (H + 1) * H * 4)) != CN_OK) return fail(st);
     d.kreal = (const float*)p;
+    if (seg_m((int)N) > 0) {
+      if ((st = dalloc(e, c.allocs, &p, K * (size_t)(seg_m((int)N) / 2 + 1) * (H + 1) * 4)) != CN_OK) return fail(st);
+      d.kseg = (const float*)p;
+    }
+    c.seg_r.assign(K, -1);
     if ((st = dalloc(e, c.allocs, &p, K * N * H * 8)) != CN_OK) return fail(st);
     d.uhat = p;
     if ((st = dalloc(e, c.allocs, &p, C * P * 4)) != CN_OK) return fail(st);
@@ -1064,9 +1176,9 @@ cn_status cn_creature_set_kernel(cn_engine* e, uint32_t i, uint32_t k, const flo
     const CreatureDev before = c.dev;
     c.dev.ksym[k] = sym ? 1 : 0;
     bool restored = false;
-    return commit_desc(e, (int)i, before, &restored);
+    if ((st = commit_desc(e, (int)i, before, &restored)) != CN_OK) return st;
   }
-  return CN_OK;
+  return build_segments(e, (int)i, {(int)k});
   CN_CATCH
 }
 
@@ -1179,7 +1291,9 @@ cn_status cn_creature_generate(cn_engine* e, uint32_t i, const cn_creature_param
   CN_HIP(hipStreamSynchronize(e->stream));
   e->have_prev = false;
   e->res.fwd_valid = false;
-  return CN_OK;
+  std::vector<int> every(c.K);
+  for (uint32_t k = 0; k < c.K; ++k) every[k] = (int)k;
+  return build_segments(e, (int)i, every);
   CN_CATCH
 }
 
@@ -1570,6 +1684,9 @@ cn_status cn_engine_get_info(const cn_engine* e, cn_engine_info* out) {
   out->device_bytes = e->bytes;
   out->compact_kernels = 0;
   for (uint32_t k = 0; k < e->cr[0].K; ++k) out->compact_kernels += e->cr[0].dev.ksym[k] ? 1u : 0u;
+  out->segments = (uint32_t)e->cr[0].dev.seg_g;
+  out->segment_radius = (uint32_t)e->cr[0].dev.seg_r;
+  out->segment_length = e->cr[0].dev.seg_g > 0 ? (uint32_t)seg_m(e->cr[0].dev.N) : 0u;
   return CN_OK;
   CN_CATCH
 }
@@ -1916,6 +2033,7 @@ bool col_geom_ok(int N, int ce, int tc) {
 struct Geom {
   int fwd_e, inv_e, col_e, col_tc, diff_th;
   bool col_split;
+  bool col_full = false;
 };
 
 // Launches one step of size class k with geometry g and events around its kernels; returns
@@ -1932,6 +2050,9 @@ cn_status time_class_step(cn_engine* e, SizeClass& k, const Geom& g, float ms[4]
   sl.col_tc = g.col_tc;
   sl.diff_th = g.diff_th;
   sl.col_split = g.col_split;
+  SizeClass kg = k;
+  kg.col_full = g.col_full;
+  set_seg_grid(e, kg, 0, (int)k.members.size(), sl);
   CN_HIP(launch_step(k.N, sl, e->stream));
   for (int q = 0; q < 5; ++q) sl.ev[q] = e->ev[q];
   CN_HIP(launch_step(k.N, sl, e->stream));
@@ -1951,13 +2072,23 @@ cn_status tune_class(cn_engine* e, SizeClass& k, uint32_t reps) {
   struct ColGeom {
     int e, tc;
     bool split;
+    bool full;
   };
+  // Full-length column geometries, and the segmented pass where some creature has segments
+  // (creatures without run the default full-length geometry beside it).
   std::vector<ColGeom> cols;
   for (bool split : {false, true}) {
     if (split && k.max_k <= k.max_c) continue;  // one kernel per source: the same launch
     for (int ce : {8, 16})
       for (int tc : {2, 4, 8, 16})
-        if (col_geom_ok(N, ce, tc)) cols.push_back({ce, tc, split});
+        if (col_geom_ok(N, ce, tc)) cols.push_back({ce, tc, split, true});
+  }
+  {
+    StepLaunch probe;
+    SizeClass kg = k;
+    kg.col_full = false;
+    set_seg_grid(e, kg, 0, (int)k.members.size(), probe);
+    if (probe.seg_g > 0) cols.push_back({0, 0, false, false});
   }
   const std::vector<int> diffs = {8, 16, 32};
   std::vector<std::vector<float>> tf(rows.size()), ti(rows.size()), tcol(cols.size()), td(diffs.size());
@@ -1966,7 +2097,8 @@ cn_status tune_class(cn_engine* e, SizeClass& k, uint32_t reps) {
   for (size_t r = 0; r < rounds * (reps + 1); ++r) {
     const size_t i = r % rounds;
     const ColGeom& cg = cols[i % cols.size()];
-    const Geom g{rows[i % rows.size()], rows[(i + 1) % rows.size()], cg.e, cg.tc, diffs[i % diffs.size()], cg.split};
+    const Geom g{rows[i % rows.size()], rows[(i + 1) % rows.size()], cg.e, cg.tc, diffs[i % diffs.size()], cg.split,
+                 cg.full};
     float ms[4];
     const cn_status st = time_class_step(e, k, g, ms);
     if (st != CN_OK) return st;
@@ -1992,7 +2124,8 @@ cn_status tune_class(cn_engine* e, SizeClass& k, uint32_t reps) {
   for (size_t c : rc)
     for (size_t f : rf)
       for (size_t i : ri)
-        for (size_t d : rd) combos.push_back({rows[f], rows[i], cols[c].e, cols[c].tc, diffs[d], cols[c].split});
+        for (size_t d : rd)
+          combos.push_back({rows[f], rows[i], cols[c].e, cols[c].tc, diffs[d], cols[c].split, cols[c].full});
   std::vector<std::vector<float>> tstep(combos.size());
   for (uint32_t r = 0; r < reps; ++r)
     for (size_t j = 0; j < combos.size(); ++j) {
@@ -2007,6 +2140,7 @@ cn_status tune_class(cn_engine* e, SizeClass& k, uint32_t reps) {
   k.col_e = g.col_e;
   k.col_tc = g.col_tc;
   k.col_split = g.col_split;
+  k.col_full = g.col_full;
   k.diff_th = g.diff_th;
   return CN_OK;
 }
@@ -2048,6 +2182,7 @@ cn_status cn_engine_get_plan(const cn_engine* e, uint32_t size_class, cn_plan* o
   out->col_tc = (uint32_t)k.col_tc;
   out->diff_th = (uint32_t)k.diff_th;
   out->col_split = k.col_split ? 1u : 0u;
+  out->col_full = k.col_full ? 1u : 0u;
   return CN_OK;
   CN_CATCH
 }
@@ -2066,6 +2201,7 @@ cn_status cn_engine_set_plan(cn_engine* e, const cn_plan* plan) {
   if (plan->diff_th != 0 && plan->diff_th != 8 && plan->diff_th != 16 && plan->diff_th != 32)
     return set_error(CN_ERR_INVALID_ARG, "diffusion tile height must be 0, 8, 16 or 32");
   if (plan->col_split > 1) return set_error(CN_ERR_INVALID_ARG, "col_split must be 0 or 1");
+  if (plan->col_full > 1) return set_error(CN_ERR_INVALID_ARG, "col_full must be 0 or 1");
   if (plan->col_tc != 0 || plan->col_e != 0) {
     const int ce = plan->col_e ? (int)plan->col_e : 16;  // the launcher's defaults (DefaultGeom)
     const int tc = plan->col_tc ? (int)plan->col_tc : default_col_tc(k->N);
@@ -2078,6 +2214,7 @@ cn_status cn_engine_set_plan(cn_engine* e, const cn_plan* plan) {
   k->col_tc = (int)plan->col_tc;
   k->diff_th = (int)plan->diff_th;
   k->col_split = plan->col_split != 0;
+  k->col_full = plan->col_full != 0;
   CN_HIP(hipSetDevice(e->device));
   CN_HIP(hipStreamSynchronize(e->stream));
   drop_graphs(e);

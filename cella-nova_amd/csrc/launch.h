@@ -1,6 +1,7 @@
 // Host-side launch interface of the device kernels (shared by the engine and the batch path).
 #pragma once
 #include <hip/hip_runtime.h>
+#include "cfg.h"
 #include "prng.h"
 #include "types.h"
 
@@ -24,6 +25,10 @@ struct StepLaunch {
   // inverse (C2R) row FFTs and of the column FFTs, column tile width, diffusion tile height.
   int fwd_e = 0, inv_e = 0, col_e = 0, col_tc = 0, diff_th = 0;
   bool col_split = false;              // column pass per (tile, kernel) instead of (tile, source)
+  // > 0: the segmented column pass over seg_g segments per column (the largest desc.seg_g of
+  // the launch; every descriptor has seg_g > 0), instead of full-length column transforms.
+  int seg_g = 0;
+  bool seg_mixed = false;              // some descriptors have seg_g = 0: full-length pass for them
   // Optional per-kernel timing events: ev[0] before the row pass, ev[1] after it, ev[2] after
   // the column pass, ev[3] after the inverse-row + growth pass, ev[4] after the diffusion.
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -60,6 +65,19 @@ hipError_t launch_gather_grads(const CreatureDev* descs, int count, float* out, 
 
 hipError_t launch_mass(const CreatureDev* descs, int count, int slot, int max_c, double2* partial, int nblk,
                        hipStream_t st);
+
+// Segmented column pass set-up, for the nk kernels ks[0..nk) of creature `desc` (grid size N,
+// real, even spectra):
+//   launch_seg_profile: column profiles h(j, kx), j = 0 .. N/2 (FP64), of every column
+//     kx = 0 .. N/2: hbuf[kk][kx][j] for j <= kSegRcap, and the tail energies
+//     etail[kk][kx][r + 1] = sum_{j > r} w_j h(j, kx)^2 for r = -1 .. kSegRcap;
+//   launch_seg_tail_sum: esum[kk][r + 1] = sum over kx of etail (fixed order);
+//   launch_seg_table: kseg rows of kernel ks[kk] from hbuf, truncated at radius rs[kk].
+hipError_t launch_seg_profile(int N, const CreatureDev* desc, const int* ks, int nk, double* hbuf, double* etail,
+                              hipStream_t st);
+hipError_t launch_seg_tail_sum(int N, const double* etail, int nk, double* esum, hipStream_t st);
+hipError_t launch_seg_table(int N, const double* hbuf, const int* ks, const int* rs, int nk, float* kseg,
+                            hipStream_t st);
 
 // Per-channel byte sums of a packed 0x00RRGGBB frame of `cells` pixels (a multiple of 4):
 // kStatsBlocks blocks each write [sum r, g, b, sum r^2, g^2, b^2] to partial[6 * block ...].

@@ -75,11 +75,11 @@ k_row_fwd(const CreatureDev* __restrict__ descs, int sin, const float2* __restri
 // ---------------------------------------------------------------------------------------
 template <int N, int TC, int EC, bool SPLIT>
 __global__ void __launch_bounds__((TC * ColCfg<N, EC>::T)) __attribute__((amdgpu_waves_per_eu(4)))
-k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) {
+k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw, int skip_seg) {
   constexpr int L = N, E = ColCfg<N, EC>::E, HALF = N / 2;
   constexpr uint32_t kPlaneBytes = (uint32_t)N * HALF * 8, kRealBytes = (uint32_t)(HALF + 1) * HALF * 4;
   const CreatureDev& d = descs[blockIdx.z];
-  if (d.N != N) return;
+  if (d.N != N || (skip_seg && d.seg_g > 0)) return;  // skip_seg: the segmented pass covers those
   int c, kb, ke;
   if constexpr (SPLIT) {
     if ((int)blockIdx.y >= d.K) return;
@@ -223,6 +223,111 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
     const BufRes uo = make_res((const char*)d.uhat + (size_t)k * kPlaneBytes, kPlaneBytes);
 #pragma unroll
     for (int i = 0; i < E; ++i) buf_store_c64(w[i], uo, vct, (uint32_t)out_pos<L, E, true>(0, i) * kRowStride);
+    __builtin_amdgcn_s_setprio(0);
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Segmented column pass (overlap-save along y), for creatures whose kernels all have real, even
+// spectra with column profiles of support |y| <= R (segment.hip): the convolution of each
+// spectrum column with its profile is computed on G segments of M rows.  Segment s produces the
+// output rows [o_s, o_{s+1}), o_s = s N / G, from the M input rows starting at o_s - R (rows
+// wrap around the torus): forward M-point transform, product with the kernel's segment table
+// H_k(m, kx) (real, even in m; the 1/M of the inverse folded in), inverse transform, and of its
+// M outputs the rows R .. R + o_{s+1} - o_s - 1 are stored (the others hold wrapped terms).
+// A block takes 16 columns of one segment of one source: its loads and stores are whole
+// 128-byte lines (16 consecutive complex values per row), and its job is short, so that the
+// blocks resident on a CU are at different phases of their jobs.
+//
+// Column kx = 0 packs the DC and Nyquist columns (X_0, X_{N/2}) as z = a + i b: with Z its
+// transform, A H0 + i B HN = Z (H0 + HN)/2 + conj(Z(-m)) (H0 - HN)/2 (H0, HN real), the mirror
+// Z(-m) taken through the exchange buffer; HN is the table's column N/2.
+// ---------------------------------------------------------------------------------------
+template <int N, int M, int E>
+__global__ void __launch_bounds__((kSegTC * (M / E))) __attribute__((amdgpu_waves_per_eu(4)))
+k_col_segments(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) {
+  constexpr int TC = kSegTC, HALF = N / 2, W = HALF + 1, NT = HALF / TC, MH = M / 2;
+  constexpr uint32_t kPlaneBytes = (uint32_t)N * HALF * 8, kTabBytes = (uint32_t)(MH + 1) * W * 4;
+  using LM = LdsMap<M, TC, true>;
+  const CreatureDev& d = descs[blockIdx.z];
+  if (d.N != N) return;
+  const int c = blockIdx.y;
+  if (c >= d.C) return;
+  const int kb = d.src_off[c], ke = d.src_off[c + 1];
+  if (kb == ke) return;
+  const int G = d.seg_g, R = d.seg_r;
+  // Jobs (tile, segment), segments fastest, dealt so that each XCD takes a contiguous run: the
+  // rows two neighbouring segments share are read through the same L2.
+  const int nb = (int)gridDim.x, bx = (int)blockIdx.x, gmax = nb / NT;
+  const int job = (nb % 8 == 0) ? (bx % 8) * (nb / 8) + bx / 8 : bx;
+  const int tile = job / gmax, seg = job % gmax;
+  if (seg >= G) return;
+  const int lane = (int)threadIdx.x % TC, x = tile * TC + lane;
+  int t = (int)threadIdx.x / TC;
+  const int o0 = seg * N / G, rows = (seg + 1) * N / G - o0, base = o0 - R;
+
+  const BufRes sp = make_res((const char*)d.spec + (size_t)c * kPlaneBytes, kPlaneBytes);
+  float2 v[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int y = (base + in_pos<M, E, false>(t, i)) & (N - 1);
+    v[i] = buf_load_c64(sp, (uint32_t)(y * HALF + x) * 8u, 0);
+  }
+  // Table factor of slot i (frequency m = out_pos(t, i), row min(m, M - m)).
+  auto load_h = [&](int k, float* out) {
+    const BufRes kr = make_res(d.kseg + (size_t)k * (MH + 1) * W, kTabBytes);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int m = out_pos<M, E, false>(t, i), mm = m <= MH ? m : M - m;
+      out[i] = buf_load_f32(kr, (uint32_t)(mm * W + x) * 4u, 0);
+    }
+  };
+  using TF = TwLds<M, E, false>;
+  using TI = TwLds<M, E, true>;
+  __shared__ float2 twl[TF::size + TI::size + 1];
+  TwFill<TF, TI, TC * (M / E)> twf;
+  twf.load(tw, threadIdx.x);
+  float hk[E];
+  load_h(d.src_k[kb], hk);
+  twf.store(twl, threadIdx.x);
+  __shared__ float2 lds[LM::size];
+  fft_run<M, E, TC, true, false, kTwN, 0, true>(v, lds, t, lane, twl);
+
+  const bool edge = tile == 0;
+  for (int kk = kb; kk < ke; ++kk) {
+    const int k = d.src_k[kk];
+    t = launder(t);
+    float2 w[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) w[i] = make_float2(v[i].x * hk[i], v[i].y * hk[i]);
+    if (edge) {
+      __syncthreads();
+      if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) lds[LM::at(out_pos<M, E, false>(t, i), 0)] = v[i];
+      }
+      __syncthreads();
+      if (lane == 0) {
+        const BufRes kr = make_res(d.kseg + (size_t)k * (MH + 1) * W, kTabBytes);
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+          const int m = out_pos<M, E, false>(t, i), mm = m <= MH ? m : M - m;
+          const float hn = buf_load_f32(kr, (uint32_t)(mm * W + HALF) * 4u, 0);
+          const float p = 0.5f * (hk[i] + hn), q = 0.5f * (hk[i] - hn);
+          const float2 F = v[i], Gm = lds[LM::at((M - m) & (M - 1), 0)];
+          w[i] = make_float2(F.x * p + Gm.x * q, F.y * p - Gm.y * q);
+        }
+      }
+    }
+    if (kk + 1 < ke) load_h(d.src_k[kk + 1], hk);
+    __builtin_amdgcn_s_setprio(1);
+    fft_run<M, E, TC, true, true, kTwN, 0, true>(w, lds, t, lane, twl + TF::size);
+    const BufRes uo = make_res((const char*)d.uhat + (size_t)k * kPlaneBytes, kPlaneBytes);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int q = out_pos<M, E, true>(t, i) - R;
+      if (q >= 0 && q < rows) buf_store_c64(w[i], uo, (uint32_t)res_index(N, o0 + q, x) * 8u, 0);
+    }
     __builtin_amdgcn_s_setprio(0);
   }
 }
@@ -627,12 +732,13 @@ static hipError_t launch_col(const StepLaunch& sl, hipStream_t st) {
   if constexpr (!col_tc_ok<N, TC, EC>()) {
     return hipErrorInvalidValue;
   } else {
+    const int skip = sl.seg_g > 0 ? 1 : 0;
     if (sl.col_split) {
       const dim3 g_col((N / 2) / TC, sl.max_k, sl.count);
-      k_col_fused<N, TC, EC, true><<<g_col, TC * ColCfg<N, EC>::T, 0, st>>>(sl.descs, sl.tw);
+      k_col_fused<N, TC, EC, true><<<g_col, TC * ColCfg<N, EC>::T, 0, st>>>(sl.descs, sl.tw, skip);
     } else {
       const dim3 g_col((N / 2) / TC, sl.max_c, sl.count);
-      k_col_fused<N, TC, EC, false><<<g_col, TC * ColCfg<N, EC>::T, 0, st>>>(sl.descs, sl.tw);
+      k_col_fused<N, TC, EC, false><<<g_col, TC * ColCfg<N, EC>::T, 0, st>>>(sl.descs, sl.tw, skip);
     }
     return hipSuccess;
   }
@@ -685,16 +791,31 @@ static hipError_t launch_rows_e(const StepLaunch& sl, hipStream_t st, bool fwd) 
 }
 
 template <int N>
+static hipError_t launch_col_segments(const StepLaunch& sl, hipStream_t st) {
+  constexpr int M = seg_m(N), E = 16;
+  if constexpr (M == 0) {
+    return hipErrorInvalidValue;
+  } else {
+    const dim3 g((N / 2 / kSegTC) * sl.seg_g, sl.max_c, sl.count);
+    k_col_segments<N, M, E><<<g, kSegTC * (M / E), 0, st>>>(sl.descs, sl.tw);
+    return hipSuccess;
+  }
+}
+
+template <int N>
 static hipError_t launch_step_n(const StepLaunch& sl, hipStream_t st) {
   hipError_t e;
   if ((e = record(sl.ev[0], st)) != hipSuccess) return e;
   if ((e = launch_rows_e<N>(sl, st, true)) != hipSuccess) return e;
   if ((e = record(sl.ev[1], st)) != hipSuccess) return e;
-  const int tc = sl.col_tc ? sl.col_tc : DefaultGeom<N>::TC;
-  switch (sl.col_e ? sl.col_e : DefaultGeom<N>::COL_E) {
-    case 8: e = launch_col_tc<N, 8>(sl, tc, st); break;
-    case 16: e = launch_col_tc<N, 16>(sl, tc, st); break;
-    default: e = hipErrorInvalidValue;
+  if (sl.seg_g > 0) e = launch_col_segments<N>(sl, st);
+  if (e == hipSuccess && (sl.seg_g == 0 || sl.seg_mixed)) {
+    const int tc = sl.col_tc ? sl.col_tc : DefaultGeom<N>::TC;
+    switch (sl.col_e ? sl.col_e : DefaultGeom<N>::COL_E) {
+      case 8: e = launch_col_tc<N, 8>(sl, tc, st); break;
+      case 16: e = launch_col_tc<N, 16>(sl, tc, st); break;
+      default: e = hipErrorInvalidValue;
+    }
   }
   if (e != hipSuccess) return e;
   if ((e = record(sl.ev[2], st)) != hipSuccess) return e;

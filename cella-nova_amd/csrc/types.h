@@ -12,6 +12,9 @@
 //                              ky (rows ky = 0 .. N/2; row N-ky equals row ky), in tiles of 4
 //                              columns (kreal_index) so that column tiles read contiguous runs;
 //                              ksym[k] = 1
+//   kseg    [K][M/2+1][N/2+1] f32 segment tables of the segmented column pass (seg_g > 0):
+//                              H_k(m, kx) = (1/M) DFT_M of kernel k's column profile truncated
+//                              to |y| <= its support, rows m = 0 .. M/2 (real, even in m)
 //   uhat    [K][N][N/2]  c64   row spectra of every convolution result (after the inverse
 //                              column pass), row-major (res_index; column tiles ct_index when
 //                              kColTiled); slot kx = 0 packs (DC, Nyquist) as (re, im)
@@ -102,6 +105,11 @@ struct CreatureDev {
   float inv_s2[kMaxKernels];   // 1 / sigma^2
   float w[kMaxKernels];
   int ksym[kMaxKernels];       // 1: kernel k is read from kreal (real, even in ky)
+  // Segmented column pass: seg_g segments of M rows (seg_m(N)) overlapping by 2 seg_r rows,
+  // for creatures whose every kernel has a real, even spectrum and column profiles within
+  // |y| <= seg_r; seg_g = 0: full-length column transforms.
+  const float* kseg;
+  int seg_g, seg_r;
 };
 
 // Offset (in complex elements) of row y, column x of a column-tiled half-spectrum plane

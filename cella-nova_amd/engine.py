@@ -282,9 +282,18 @@ class _Engine:
             out.append({f: getattr(p, f) for f, _ in Plan._fields_})
         return out
 
-    def set_plan(self, n, fwd_e=0, inv_e=0, col_e=0, col_tc=0, diff_th=0, col_split=0):
-        p = Plan(int(n), int(fwd_e), int(inv_e), int(col_e), int(col_tc), int(diff_th), int(col_split))
+    def set_plan(self, n, fwd_e=0, inv_e=0, col_e=0, col_tc=0, diff_th=0, col_split=0, col_full=0):
+        p = Plan(int(n), int(fwd_e), int(inv_e), int(col_e), int(col_tc), int(diff_th), int(col_split), int(col_full))
         check(lib().cn_engine_set_plan(self._h, ctypes.byref(p)))
+
+    def info(self):
+        """Engine summary; per-creature fields (n, channels, kernels, compact kernels, segments)
+        describe creature 0."""
+        i = EngineInfo()
+        check(lib().cn_engine_get_info(self._h, ctypes.byref(i)))
+        return {"n": i.n, "num_channels": i.num_channels, "num_kernels": i.num_kernels,
+                "steps_done": i.steps_done, "device_bytes": i.device_bytes, "compact_kernels": i.compact_kernels,
+                "segments": i.segments, "segment_radius": i.segment_radius, "segment_length": i.segment_length}
 
     def profile_kernels(self, steps):
         """Average device ms per launch over `steps` directly launched steps (first size class):
@@ -356,11 +365,6 @@ class GpuMaceLenia(_Engine):
         """Per-channel FP64 compensated mass of the current state."""
         return self.creature_mass(0)
 
-    def info(self):
-        i = EngineInfo()
-        check(lib().cn_engine_get_info(self._h, ctypes.byref(i)))
-        return {"n": i.n, "num_channels": i.num_channels, "num_kernels": i.num_kernels,
-                "steps_done": i.steps_done, "device_bytes": i.device_bytes, "compact_kernels": i.compact_kernels}
 
 
 class CreatureBatch(_Engine):

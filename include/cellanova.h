@@ -144,6 +144,11 @@ typedef struct cn_engine_info {
   uint64_t device_bytes;   /* device memory held by the engine */
   uint32_t compact_kernels; /* kernels of creature 0 whose spectrum is real and even in ky and
                                is read from the compact real copy (a quarter of the bytes) */
+  uint32_t segments;        /* creature 0's segmented column pass: segments per column (0: its
+                               column transforms are full-length) */
+  uint32_t segment_radius;  /* the largest column-profile support radius of its kernels: the
+                               segments overlap by twice this many rows */
+  uint32_t segment_length;  /* rows per segment (the segment transform length) */
 } cn_engine_info;
 cn_status cn_engine_get_info(const cn_engine* e, cn_engine_info* out);
 
@@ -245,6 +250,10 @@ typedef struct cn_plan {
   uint32_t n;
   uint32_t fwd_e, inv_e, col_e, col_tc, diff_th;
   uint32_t col_split;
+  /* 1: full-length column transforms for every creature of the class; 0: the segmented column
+   * pass for creatures whose kernels have real, even spectra of compact column support (see
+   * cn_engine_info's segments), full-length transforms for the others. */
+  uint32_t col_full;
 } cn_plan;
 
 /* Times every candidate geometry of every size class on the device (reps timed launches per

@@ -1,0 +1,149 @@
+"""GPU: the segmented column pass (overlap-save along the columns, k_col_segments) against the
+full-length column transforms and the FP64 oracle.
+
+A kernel's column profile h(y, kx) = sum_ky Khat_h(ky, kx) e^{2 pi i ky y / N} has the kernel's
+support in y; the engine finds it from the stored spectrum (tail energy within (2^-22)^2 of the
+total, in FP64) and convolves each column on segments of 256 rows that overlap by twice that
+radius.  Checked here: the radius against the same criterion computed with numpy; the
+convolution results against the full-length path of the same creature (plan col_full) and
+against the FP64 oracle; a kernel without compact support (a wide Gaussian) falls back to the
+full-length path, and back again; a batch that mixes both kinds of creatures steps each of them
+exactly as its own engine does.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N = 2048
+
+
+def _hermitian(kf, n):
+    K = kf.reshape(n, n).astype(np.complex128)
+    Km = np.conj(np.roll(np.flip(K, (0, 1)), 1, (0, 1)))
+    return (0.5 * (K + Km) / n**2).astype(np.complex64).astype(np.complex128)
+
+
+def _numpy_radius(kfs, n, cap=64):
+    """The largest support radius over the kernels (None if one has none within `cap`)."""
+    out = -1
+    j = np.arange(n)
+    dist = np.minimum(j, n - j)
+    for kf in kfs:
+        Kr = _hermitian(kf, n).real[:, : n // 2 + 1]
+        h = np.fft.ifft(Kr, axis=0).real * n
+        e = h**2
+        tot = e.sum()
+        r = next((r for r in range(cap + 1) if e[dist > r].sum() <= 2.0**-44 * tot), None)
+        if r is None:
+            return None
+        out = max(out, r)
+    return out
+
+
+def _wide_gaussian(n, sigma=200.0):
+    d = np.minimum(np.arange(n), n - np.arange(n)).astype(np.float64)
+    k = np.exp(-(d[:, None] ** 2 + d[None, :] ** 2) / (2 * sigma**2))
+    return np.fft.fft2(k / k.sum()).astype(np.complex64).reshape(-1)
+
+
+@pytest.fixture(scope="module")
+def creature(cn):
+    return cn.baseline_creature(N, num_channels=3, radius=26, seed=0)
+
+
+def test_radius_and_segments_match_numpy(cn, creature):
+    eng = cn.make_engine(creature)
+    info = eng.info()
+    eng.close()
+    R = _numpy_radius(creature["kernels_fft"], N)
+    assert R is not None and R < 26
+    assert info["segment_radius"] == R
+    assert info["segments"] == -(-N // (256 - 2 * R))
+
+
+def test_segmented_convolution_matches_full_path_and_oracle(cn, orc, creature):
+    seg = cn.make_engine(creature)
+    full = cn.make_engine(creature)
+    full.set_plan(N, col_full=1)
+    assert seg.info()["segments"] > 0
+    K = creature["num_kernels"]
+    seg.iterate()
+    full.iterate()
+    # the convolution buffer of the step just taken, unnormalised like the reference's
+    us = seg.download_conv_buffer().reshape(K, N, N) / N**2
+    uf = full.download_conv_buffer().reshape(K, N, N) / N**2
+    want_all = orc.conv_f64(creature["state"].astype(np.float64), creature["kernels_fft"], creature["src"])
+    for k in range(K):
+        want = want_all[k]
+        es = orc.rel_linf(us[k], want)
+        ef = orc.rel_linf(uf[k], want)
+        print(f"kernel {k}: segmented {es:.2e}, full-length {ef:.2e} (rel L-inf vs FP64)")
+        assert es <= 2e-6 and ef <= 2e-6
+        assert es <= 2 * ef + 2e-7
+    # steps: both paths stay within the step bar of each other over a short run
+    seg.run_steps(20)
+    full.run_steps(20)
+    assert orc.rel_linf(seg.download_all_channels(), full.download_all_channels().astype(np.float64)) <= 1e-5
+    seg.close()
+    full.close()
+
+
+def test_non_compact_kernel_falls_back_and_back(cn, creature):
+    kfs = creature["kernels_fft"]
+    eng = cn.make_engine(creature)
+    eng.run_steps(3)  # graphs captured with the segmented pass
+    eng.set_kernel(_wide_gaussian(N), 4)
+    assert eng.info()["segments"] == 0 and eng.info()["compact_kernels"] == 9
+    eng.run_steps(2)
+    s_mixed = eng.download_all_channels()
+    eng.set_kernel(kfs[4], 4)
+    assert eng.info()["segments"] > 0
+    eng.run_steps(2)
+    got = eng.download_all_channels()
+    eng.close()
+    # the same sequence on engines built for each phase: bit for bit
+    ref = cn.make_engine(creature)
+    ref.run_steps(3)
+    state3 = ref.download_all_channels()
+    ref.close()
+    cr_g = dict(creature)
+    cr_g["kernels_fft"] = np.array(kfs).copy()
+    cr_g["kernels_fft"][4] = _wide_gaussian(N)
+    cr_g["state"] = state3.reshape(creature["state"].shape)
+    e2 = cn.make_engine(cr_g)
+    e2.run_steps(2)
+    assert np.array_equal(e2.download_all_channels(), s_mixed)
+    e2.close()
+    cr_b = dict(creature)
+    cr_b["state"] = s_mixed.reshape(creature["state"].shape)
+    e3 = cn.make_engine(cr_b)
+    e3.run_steps(2)
+    assert np.array_equal(e3.download_all_channels(), got)
+    e3.close()
+
+
+def test_mixed_batch_steps_each_creature_as_its_engine(cn, creature):
+    other = dict(creature)
+    other["kernels_fft"] = np.array(creature["kernels_fft"]).copy()
+    other["kernels_fft"][2] = _wide_gaussian(N)
+    crs = [creature, other]
+    b = cn.CreatureBatch(crs)
+    for i, cr in enumerate(crs):
+        b.load(i, cr)
+    b.run_steps(3)
+    for i, cr in enumerate(crs):
+        e = cn.make_engine(cr)
+        assert (e.info()["segments"] > 0) == (i == 0)
+        e.run_steps(3)
+        assert np.array_equal(b.creature_download(i), e.download_all_channels()), i
+        e.close()
+    b.close()
+
+
+def test_generated_creature_has_segments(cn):
+    spec = cn.creature_spec(N, 3, 26, seed=0)
+    b = cn.CreatureBatch.generated([spec])
+    info = b.info()
+    b.close()
+    assert info["segments"] > 0 and info["segment_radius"] < 26
