@@ -57,6 +57,8 @@ __device__ __forceinline__ int xcd_tile(int bx) {
 // (0: full-length column transforms only).
 constexpr int kSegTC = 16;
 constexpr int kSegRcap = 64;
-constexpr int seg_m(int N) { return N == 2048 ? 256 : 0; }
+constexpr int seg_m(int N) { return N == 2048 ? 256 : (N == 1024 || N == 512) ? 128 : 0; }
+// The largest support radius a segment takes: half of its rows stay valid outputs.
+constexpr int seg_rmax(int N) { return seg_m(N) / 4 < kSegRcap ? seg_m(N) / 4 : kSegRcap; }
 
 }  // namespace cn

@@ -484,6 +484,7 @@ cn_status build_segments(cn_engine* e, int i, const std::vector<int>& ks) {
     all = all && r >= 0;
     R = std::max(R, r);
   }
+  all = all && R <= seg_rmax(N);
   const int g = all ? (N + (M - 2 * R) - 1) / (M - 2 * R) : 0, r = all ? R : 0;
   if (g != c.dev.seg_g || r != c.dev.seg_r) {
     const CreatureDev before = c.dev;

@@ -49,15 +49,22 @@ k_seg_profile(const CreatureDev* __restrict__ desc, const int* __restrict__ ks, 
   }
 }
 
+// One block per (radius, kernel): strided partial sums over the columns, then a fixed-shape
+// tree in LDS (the same order on every run).
 __global__ void __launch_bounds__(256)
-k_seg_tail_sum(const double* __restrict__ etail, int nk, int W, double* __restrict__ esum) {
+k_seg_tail_sum(const double* __restrict__ etail, int W, double* __restrict__ esum) {
   constexpr int RC = kSegRcap;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nk * (RC + 2)) return;
-  const int kk = i / (RC + 2), r1 = i % (RC + 2);
+  const int r1 = blockIdx.x, kk = blockIdx.y;
+  __shared__ double part[256];
   double e = 0.0;
-  for (int kx = 0; kx < W; ++kx) e += etail[((size_t)kk * W + kx) * (RC + 2) + r1];
-  esum[i] = e;
+  for (int kx = threadIdx.x; kx < W; kx += 256) e += etail[((size_t)kk * W + kx) * (RC + 2) + r1];
+  part[threadIdx.x] = e;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) esum[kk * (RC + 2) + r1] = part[0];
 }
 
 template <int M>
@@ -79,14 +86,15 @@ hipError_t launch_seg_profile(int N, const CreatureDev* desc, const int* ks, int
   const dim3 g(N / 2 + 1, nk);
   switch (N) {
     case 2048: k_seg_profile<2048><<<g, 256, 0, st>>>(desc, ks, hbuf, etail); break;
+    case 1024: k_seg_profile<1024><<<g, 256, 0, st>>>(desc, ks, hbuf, etail); break;
+    case 512: k_seg_profile<512><<<g, 256, 0, st>>>(desc, ks, hbuf, etail); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
 hipError_t launch_seg_tail_sum(int N, const double* etail, int nk, double* esum, hipStream_t st) {
-  const int n = nk * (kSegRcap + 2);
-  k_seg_tail_sum<<<(n + 255) / 256, 256, 0, st>>>(etail, nk, N / 2 + 1, esum);
+  k_seg_tail_sum<<<dim3(kSegRcap + 2, nk), 256, 0, st>>>(etail, N / 2 + 1, esum);
   return hipGetLastError();
 }
 
@@ -95,6 +103,7 @@ hipError_t launch_seg_table(int N, const double* hbuf, const int* ks, const int*
   const dim3 g(N / 2 + 1, nk);
   switch (seg_m(N)) {
     case 256: k_seg_table<256><<<g, 256, 0, st>>>(hbuf, ks, rs, N / 2 + 1, kseg); break;
+    case 128: k_seg_table<128><<<g, 256, 0, st>>>(hbuf, ks, rs, N / 2 + 1, kseg); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -19,7 +19,8 @@ import json
 import os
 import sys
 
-SHORT = {"k_row_fwd": "row_fwd_r2c", "k_col_fused": "col_fused", "k_row_inv_growth": "row_inv_growth",
+SHORT = {"k_row_fwd": "row_fwd_r2c", "k_col_fused": "col_fused", "k_col_segments": "col_fused",
+         "k_row_inv_growth": "row_inv_growth",
          "k_row_inv_growth_pair": "row_inv_growth",
          "k_diffusion": "diffusion", "k_diffusion_adj": "diffusion_adj", "k_growth_adj_row": "growth_adj_row",
          "k_col_adj": "col_adj", "k_row_inv_add": "row_inv_add", "k_param_reduce": "param_reduce",

@@ -52,28 +52,41 @@ def creature(cn):
     return cn.baseline_creature(N, num_channels=3, radius=26, seed=0)
 
 
-def test_radius_and_segments_match_numpy(cn, creature):
-    eng = cn.make_engine(creature)
+# (grid, channels, ring radius, segment length): the 2048^2 step, the 1024^2 rollout and the
+# 512^2 C5K25 search shapes
+SHAPES = [(2048, 3, 26, 256), (1024, 3, 13, 128), (512, 5, 12, 128)]
+
+
+@pytest.mark.parametrize("n,C,R0,M", SHAPES)
+def test_radius_and_segments_match_numpy(cn, n, C, R0, M):
+    cr = creature_of(cn, n, C, R0)
+    eng = cn.make_engine(cr)
     info = eng.info()
     eng.close()
-    R = _numpy_radius(creature["kernels_fft"], N)
-    assert R is not None and R < 26
-    assert info["segment_radius"] == R
-    assert info["segments"] == -(-N // (256 - 2 * R))
+    R = _numpy_radius(cr["kernels_fft"], n)
+    assert R is not None and R < R0
+    assert info["segment_radius"] == R and info["segment_length"] == M
+    assert info["segments"] == -(-n // (M - 2 * R))
 
 
-def test_segmented_convolution_matches_full_path_and_oracle(cn, orc, creature):
-    seg = cn.make_engine(creature)
-    full = cn.make_engine(creature)
-    full.set_plan(N, col_full=1)
+def creature_of(cn, n, C, R0):
+    return cn.baseline_creature(n, num_channels=C, radius=R0, seed=0, patch=min(n // 2, 256))
+
+
+@pytest.mark.parametrize("n,C,R0,M", SHAPES)
+def test_segmented_convolution_matches_full_path_and_oracle(cn, orc, n, C, R0, M):
+    cr = creature_of(cn, n, C, R0)
+    seg = cn.make_engine(cr)
+    full = cn.make_engine(cr)
+    full.set_plan(n, col_full=1)
     assert seg.info()["segments"] > 0
-    K = creature["num_kernels"]
+    K = cr["num_kernels"]
     seg.iterate()
     full.iterate()
     # the convolution buffer of the step just taken, unnormalised like the reference's
-    us = seg.download_conv_buffer().reshape(K, N, N) / N**2
-    uf = full.download_conv_buffer().reshape(K, N, N) / N**2
-    want_all = orc.conv_f64(creature["state"].astype(np.float64), creature["kernels_fft"], creature["src"])
+    us = seg.download_conv_buffer().reshape(K, n, n) / n**2
+    uf = full.download_conv_buffer().reshape(K, n, n) / n**2
+    want_all = orc.conv_f64(cr["state"].astype(np.float64), cr["kernels_fft"], cr["src"])
     for k in range(K):
         want = want_all[k]
         es = orc.rel_linf(us[k], want)
@@ -85,6 +98,11 @@ def test_segmented_convolution_matches_full_path_and_oracle(cn, orc, creature):
     seg.run_steps(20)
     full.run_steps(20)
     assert orc.rel_linf(seg.download_all_channels(), full.download_all_channels().astype(np.float64)) <= 1e-5
+    # and a step from the same state against the FP64 oracle
+    prev = seg.download_all_channels().reshape(cr["state"].shape).astype(np.float64)
+    seg.iterate()
+    want = orc.run_f64(prev, cr["kernels_fft"], cr["src"], cr["dst"], cr["mu"], cr["sigma"], cr["weights"])
+    assert orc.rel_linf(seg.download_all_channels().reshape(want.shape), want) <= 1e-5
     seg.close()
     full.close()
 
