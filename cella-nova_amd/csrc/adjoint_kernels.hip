@@ -16,6 +16,7 @@
 // k_col_adj (forward column FFT of every gu_k, product with the conjugate spectrum, summed per
 // source, one inverse column FFT), k_row_inv_add (row C2R + diffusion part), k_param_reduce.
 #include <hip/hip_runtime.h>
+#include "bufio.h"
 #include "cfg.h"
 #include "fft.h"
 #include "launch.h"
@@ -174,10 +175,10 @@ k_growth_adj_row(const CreatureDev* __restrict__ descs, const float2* __restrict
 // ---------------------------------------------------------------------------------------
 template <int N, int TC, int EC>
 __global__ void __launch_bounds__((TC * ColCfg<N, EC>::T)) __attribute__((amdgpu_waves_per_eu(4)))
-k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) {
+k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw, int skip_seg) {
   constexpr int L = N, E = ColCfg<N, EC>::E, HALF = N / 2;
   const CreatureDev& d = descs[blockIdx.z];
-  if (d.N != N) return;
+  if (d.N != N || (skip_seg && d.seg_g > 0)) return;  // skip_seg: the segmented pass covers those
   const int c = blockIdx.y;
   if (c >= d.C) return;
   const int kb = d.src_off[c], ke = d.src_off[c + 1];
@@ -262,6 +263,95 @@ k_col_adj(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) 
   gptr<float2> out = c64(d.adjspec) + (size_t)c * N * HALF;
 #pragma unroll
   for (int i = 0; i < E; ++i) out[res_index(N, out_pos<L, E, true>(t, i), x)] = acc[i];
+}
+
+// ---------------------------------------------------------------------------------------
+// Segmented adjoint column pass, for creatures with the segmented forward pass (every kernel
+// real and even, column profiles of support |y| <= seg_r): conj(Khat_h) = Khat_h, so the
+// correlation with each profile is the forward convolution, and per source the sum over its
+// kernels is taken on the segments' spectra before one inverse transform.  Jobs, segment rows,
+// tables and the packed DC/Nyquist column as in k_col_segments; a source without kernels
+// writes zeros, as k_col_adj does.
+// ---------------------------------------------------------------------------------------
+template <int N, int M, int E>
+__global__ void __launch_bounds__((kSegTC * (M / E))) __attribute__((amdgpu_waves_per_eu(4)))
+k_col_adj_segments(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) {
+  constexpr int TC = kSegTC, HALF = N / 2, W = HALF + 1, NT = HALF / TC, MH = M / 2;
+  constexpr uint32_t kPlaneBytes = (uint32_t)N * HALF * 8, kTabBytes = (uint32_t)(MH + 1) * W * 4;
+  using LM = LdsMap<M, TC, true>;
+  const CreatureDev& d = descs[blockIdx.z];
+  if (d.N != N) return;
+  const int c = blockIdx.y;
+  if (c >= d.C) return;
+  const int G = d.seg_g, R = d.seg_r;
+  const int nb = (int)gridDim.x, bx = (int)blockIdx.x, gmax = nb / NT;
+  const int job = (nb % 8 == 0) ? (bx % 8) * (nb / 8) + bx / 8 : bx;
+  const int tile = job / gmax, seg = job % gmax;
+  if (seg >= G) return;
+  const int kb = d.src_off[c], ke = d.src_off[c + 1];
+  const int lane = (int)threadIdx.x % TC, x = tile * TC + lane;
+  int t = (int)threadIdx.x / TC;
+  const int o0 = seg * N / G, rows = (seg + 1) * N / G - o0, base = o0 - R;
+  const bool edge = tile == 0;
+  using TF = TwLds<M, E, false>;
+  using TI = TwLds<M, E, true>;
+  __shared__ float2 twl[TF::size + TI::size + 1];
+  {
+    TwFill<TF, TI, TC * (M / E)> twf;
+    twf.load(tw, threadIdx.x);
+    twf.store(twl, threadIdx.x);
+  }
+  __shared__ float2 lds[LM::size];
+  float2 acc[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) acc[i] = make_float2(0.f, 0.f);
+  for (int kk = kb; kk < ke; ++kk) {
+    const int k = d.src_k[kk];
+    t = launder(t);
+    const BufRes sp = make_res((const char*)d.specg + (size_t)k * kPlaneBytes, kPlaneBytes);
+    const BufRes kr = make_res(d.kseg + (size_t)k * (MH + 1) * W, kTabBytes);
+    float2 v[E];
+    float hk[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int y = (base + in_pos<M, E, false>(t, i)) & (N - 1);
+      v[i] = buf_load_c64(sp, (uint32_t)(y * HALF + x) * 8u, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int m = out_pos<M, E, false>(t, i), mm = m <= MH ? m : M - m;
+      hk[i] = buf_load_f32(kr, (uint32_t)(mm * W + x) * 4u, 0);
+    }
+    fft_run<M, E, TC, true, false, kTwN, 0, true>(v, lds, t, lane, twl);
+    if (edge) {
+      __syncthreads();
+      if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) lds[LM::at(out_pos<M, E, false>(t, i), 0)] = v[i];
+      }
+      __syncthreads();
+    }
+    if (edge && lane == 0) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const int m = out_pos<M, E, false>(t, i), mm = m <= MH ? m : M - m;
+        const float hn = buf_load_f32(kr, (uint32_t)(mm * W + HALF) * 4u, 0);
+        const float p = 0.5f * (hk[i] + hn), q = 0.5f * (hk[i] - hn);
+        const float2 F = v[i], Gm = lds[LM::at((M - m) & (M - 1), 0)];
+        acc[i] = make_float2(acc[i].x + F.x * p + Gm.x * q, acc[i].y + F.y * p - Gm.y * q);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < E; ++i) acc[i] = make_float2(acc[i].x + v[i].x * hk[i], acc[i].y + v[i].y * hk[i]);
+    }
+  }
+  fft_run<M, E, TC, true, true, kTwN, 0, true>(acc, lds, t, lane, twl + TF::size);
+  const BufRes ao = make_res((const char*)d.adjspec + (size_t)c * kPlaneBytes, kPlaneBytes);
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int q = out_pos<M, E, true>(t, i) - R;
+    if (q >= 0 && q < rows) buf_store_c64(acc[i], ao, (uint32_t)res_index(N, o0 + q, x) * 8u, 0);
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -385,7 +475,8 @@ static hipError_t launch_col_adj(const StepLaunch& sl, hipStream_t st) {
   if constexpr (!col_tc_ok<N, TC, EC>()) {
     return hipErrorInvalidValue;
   } else {
-    k_col_adj<N, TC, EC><<<dim3((N / 2) / TC, sl.max_c, sl.count), TC * ColCfg<N, EC>::T, 0, st>>>(sl.descs, sl.tw);
+    k_col_adj<N, TC, EC><<<dim3((N / 2) / TC, sl.max_c, sl.count), TC * ColCfg<N, EC>::T, 0, st>>>(
+        sl.descs, sl.tw, sl.seg_g > 0 ? 1 : 0);
     return hipSuccess;
   }
 }
@@ -453,7 +544,14 @@ static hipError_t launch_adjoint_n(const StepLaunch& sl, int gin, int gout, hipS
   int tc = sl.col_tc ? sl.col_tc : DefaultGeom<N>::TC;
   while (tc > 2 && tc * ColCfg<N, 8>::T > 1024) tc /= 2;
   if (tc > 4) tc = 4;
-  if ((e = launch_col_adj_tc<N, 8>(sl, tc, st)) != hipSuccess) return e;
+  if constexpr (seg_m(N) > 0) {
+    if (sl.seg_g > 0) {
+      constexpr int M = seg_m(N), SE = 8;
+      k_col_adj_segments<N, M, SE><<<dim3((N / 2 / kSegTC) * sl.seg_g, sl.max_c, sl.count), kSegTC * (M / SE), 0,
+                                      st>>>(sl.descs, sl.tw);
+    }
+  }
+  if ((sl.seg_g == 0 || sl.seg_mixed) && (e = launch_col_adj_tc<N, 8>(sl, tc, st)) != hipSuccess) return e;
   if ((e = rows(false)) != hipSuccess) return e;
   // Parameter-gradient reduction over the partials of the growth adjoint, whose row geometry
   // (the plan's forward one) sets their block count; the inverse row geometry may differ.
