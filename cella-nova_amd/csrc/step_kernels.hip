@@ -235,18 +235,18 @@ k_col_fused(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw
 // wrap around the torus): forward M-point transform, product with the kernel's segment table
 // H_k(m, kx) (real, even in m; the 1/M of the inverse folded in), inverse transform, and of its
 // M outputs the rows R .. R + o_{s+1} - o_s - 1 are stored (the others hold wrapped terms).
-// A block takes 16 columns of one segment of one source: its loads and stores are whole
-// 128-byte lines (16 consecutive complex values per row), and its job is short, so that the
-// blocks resident on a CU are at different phases of their jobs.
+// A block takes seg_tc(N) columns (16 or 32) of one segment of one source: its loads and stores
+// are whole 128-byte lines (16 consecutive complex values per row), and its job is short, so
+// that the blocks resident on a CU are at different phases of their jobs.
 //
 // Column kx = 0 packs the DC and Nyquist columns (X_0, X_{N/2}) as z = a + i b: with Z its
 // transform, A H0 + i B HN = Z (H0 + HN)/2 + conj(Z(-m)) (H0 - HN)/2 (H0, HN real), the mirror
 // Z(-m) taken through the exchange buffer; HN is the table's column N/2.
 // ---------------------------------------------------------------------------------------
 template <int N, int M, int E>
-__global__ void __launch_bounds__((kSegTC * (M / E))) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__((seg_tc(N) * (M / E))) __attribute__((amdgpu_waves_per_eu(4)))
 k_col_segments(const CreatureDev* __restrict__ descs, const float2* __restrict__ tw) {
-  constexpr int TC = kSegTC, HALF = N / 2, W = HALF + 1, NT = HALF / TC, MH = M / 2;
+  constexpr int TC = seg_tc(N), HALF = N / 2, W = HALF + 1, NT = HALF / TC, MH = M / 2;
   constexpr uint32_t kPlaneBytes = (uint32_t)N * HALF * 8, kTabBytes = (uint32_t)(MH + 1) * W * 4;
   using LM = LdsMap<M, TC, true>;
   const CreatureDev& d = descs[blockIdx.z];
@@ -796,8 +796,8 @@ static hipError_t launch_col_segments(const StepLaunch& sl, hipStream_t st) {
   if constexpr (M == 0) {
     return hipErrorInvalidValue;
   } else {
-    const dim3 g((N / 2 / kSegTC) * sl.seg_g, sl.max_c, sl.count);
-    k_col_segments<N, M, E><<<g, kSegTC * (M / E), 0, st>>>(sl.descs, sl.tw);
+    const dim3 g((N / 2 / seg_tc(N)) * sl.seg_g, sl.max_c, sl.count);
+    k_col_segments<N, M, E><<<g, seg_tc(N) * (M / E), 0, st>>>(sl.descs, sl.tw);
     return hipSuccess;
   }
 }
