@@ -436,8 +436,9 @@ def main():
     ms_per_step = dt * 1e3 / args.steps
     value = n * n * args.steps / dt
 
-    # Per-kernel average launch durations (HIP events on the engine stream), same number of steps.
-    prof_steps = min(args.steps, 200)
+    # Per-kernel average launch durations: HIP events on the engine stream around one pass per
+    # step of 4 * prof_steps direct steps queued back to back (cn_profile_kernels).
+    prof_steps = min(args.steps, 50)
     kms = eng.profile_kernels(prof_steps)
     info = eng.info()
     compact = info["compact_kernels"]

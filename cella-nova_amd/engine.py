@@ -296,8 +296,9 @@ class _Engine:
                 "segments": i.segments, "segment_radius": i.segment_radius, "segment_length": i.segment_length}
 
     def profile_kernels(self, steps):
-        """Average device ms per launch over `steps` directly launched steps (first size class):
-        [row forward, fused column, inverse row + growth, diffusion]."""
+        """Average device ms per launch of each pass of the first size class over `steps` of
+        4 * steps direct steps queued back to back (events around one pass per step): [row
+        forward, column pass, inverse row + growth, diffusion].  The state advances 4 * steps."""
         out = np.empty(4, dtype=np.float64)
         check(lib().cn_profile_kernels(self._h, int(steps), out.ctypes.data, out.size))
         return out

@@ -96,10 +96,11 @@ cn_status cn_creature_state_ptr(cn_engine* e, uint32_t creature, float** d_state
  * the growth/affinity epilogue), ms[2] diffusion, ms[3] copy (0: ping-pong pointer swap). */
 cn_status cn_timed_iterate(cn_engine* e, double ms[4]);
 
-/* New (extends timed_iterate): run `steps` directly launched steps with HIP events around every
- * kernel on the engine's stream and return the average device time per launch in ms:
- * kernel_ms[0] row forward (R2C), [1] fused column pass, [2] inverse row + growth, [3] diffusion.
- * n must be 4.  Synchronises. */
+/* New (extends timed_iterate): 4 * steps directly launched steps queued back to back on the
+ * engine's stream, step s with HIP events around pass s % 4 (first size class) only; returns the
+ * average device time per launch in ms: kernel_ms[0] row forward (R2C), [1] column pass, [2]
+ * inverse row + growth, [3] diffusion.  The state advances 4 * steps steps.  n must be 4,
+ * 1 <= steps <= 10000.  Synchronises. */
 cn_status cn_profile_kernels(cn_engine* e, uint64_t steps, double* kernel_ms, size_t n);
 
 /* GpuMaceLenia::download_all_channels (mod.rs:236-263): C*N*N f32, channel-major. */

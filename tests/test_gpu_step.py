@@ -175,6 +175,24 @@ def test_graph_replay_equals_direct_launch(cn):
     b.close()
 
 
+@pytest.mark.parametrize("n", [128, 2048])
+def test_profile_kernels_steps(cn, n):
+    """cn_profile_kernels times the passes of 4 * steps direct steps: positive per-launch
+    times, and the state of as many plain steps."""
+    cr = cn.baseline_creature(n, radius=max(4, n // 80), seed=6, patch=64)
+    a, b = cn.make_engine(cr), cn.make_engine(cr)
+    ms = a.profile_kernels(7)
+    b.run_steps(28)
+    assert ms.shape == (4,) and np.all(ms > 0)
+    assert a.info()["steps_done"] == 28
+    assert np.array_equal(a.download_all_channels(), b.download_all_channels())
+    a.run_steps(3)
+    b.run_steps(3)
+    assert np.array_equal(a.download_all_channels(), b.download_all_channels())
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("n,fwd_e", [(128, 0), (1024, 8), (1024, 16)])
 def test_multi_step_graph_equals_single_steps(cn, n, fwd_e):
     """run_steps replays graphs of several steps for long runs: the same states as single
