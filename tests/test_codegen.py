@@ -17,8 +17,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "cella-nova_amd")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HOT = ("k_row_fwd", "k_col_fused", "k_row_inv_growth", "k_diffusion", "k_diffusion_adj", "k_growth_adj_row",
-       "k_col_adj", "k_row_inv_add", "k_gen_rows", "k_gen_cols")
+HOT = ("k_row_fwd", "k_col_fused", "k_col_segments", "k_row_inv_growth", "k_diffusion", "k_diffusion_adj",
+       "k_growth_adj_row", "k_col_adj", "k_row_inv_add", "k_gen_rows", "k_gen_cols")
 
 
 _ASM = {}
