@@ -354,16 +354,66 @@ k_col_adj_segments(const CreatureDev* __restrict__ descs, const float2* __restri
   }
 }
 
+constexpr int kReduceWaves = 4;
+
+// One (kernel, parameter) pair idx = 3 k + j of creature d, on one full wavefront (lane).
+__device__ __forceinline__ void param_reduce_pair(const CreatureDev& d, int idx, int lane, int nblk) {
+  const int K = d.K;
+  const int k = idx / 3, j = idx % 3;
+  gptr<const float> part = to_global((const float*)d.partial);
+  float s = 0.f, comp = 0.f;  // running sum and Kahan compensation (true sum = s - comp)
+  for (int b = lane; b < nblk; b += 64) {
+    const float yv = part[((size_t)b * K + k) * 3 + j] - comp;
+    const float tt = s + yv;
+    comp = (tt - s) - yv;
+    s = tt;
+  }
+  for (int m = 32; m >= 1; m >>= 1) {
+    const float s2 = __shfl_xor(s, m, 64), c2 = __shfl_xor(comp, m, 64);
+    const float t = s + s2;  // TwoSum: s + s2 = t + e exactly
+    const float bv = t - s;
+    const float e = (s - (t - bv)) + (s2 - bv);
+    comp = (comp + c2) - e;
+    s = t;
+  }
+  if (lane == 0) {
+    gptr<float> acc = to_global(d.gparam) + j * K + k;
+    gptr<float> cmp = to_global(d.gparam) + 3 * K + j * K + k;
+    const float yv = (s - comp) - *cmp;
+    const float tt = *acc + yv;
+    *cmp = (tt - *acc) - yv;
+    *acc = tt;
+  }
+}
+
+__global__ void __launch_bounds__(64 * kReduceWaves)
+k_param_reduce(const CreatureDev* __restrict__ descs, int nblk) {
+  const CreatureDev& d = descs[blockIdx.z];
+  const int idx = (int)blockIdx.x * kReduceWaves + (int)(threadIdx.x >> 6);
+  if (idx >= 3 * d.K) return;  // a whole wavefront; the kernel has no barriers
+  param_reduce_pair(d, idx, (int)(threadIdx.x & 63), nblk);
+}
+
 // ---------------------------------------------------------------------------------------
 // Row C2R of adjspec[c] (the convolution part of dL/ds) plus the diffusion part -> grad slot.
 // ---------------------------------------------------------------------------------------
 template <int N, int ER>
 __global__ void __launch_bounds__((RowCfg<N, ER>::THREADS))
-k_row_inv_add(const CreatureDev* __restrict__ descs, int gout, const float2* __restrict__ tw) {
+k_row_inv_add(const CreatureDev* __restrict__ descs, int gout, const float2* __restrict__ tw, int red_nblk) {
   using CF = RowCfg<N, ER>;
   constexpr int L = CF::L, E = CF::E, T = CF::T, ROWS = CF::ROWS;
   const CreatureDev& d = descs[blockIdx.z];
   if (d.N != N) return;
+  if (red_nblk > 0 && (int)blockIdx.y == (int)gridDim.y - 1) {
+    // The last y slice reduces the growth adjoint's parameter partials (red_nblk per kernel),
+    // which are final once the growth adjoint has run: one launch fewer per adjoint step.
+    if constexpr (CF::THREADS % 64 == 0) {
+      constexpr int W = CF::THREADS / 64;
+      for (int idx = (int)blockIdx.x * W + (int)(threadIdx.x >> 6); idx < 3 * d.K; idx += (int)gridDim.x * W)
+        param_reduce_pair(d, idx, (int)(threadIdx.x & 63), red_nblk);
+    }
+    return;
+  }
   const int c = blockIdx.y;
   if (c >= d.C) return;
   const int lane = threadIdx.x / T, t = threadIdx.x % T;
@@ -396,41 +446,6 @@ k_row_inv_add(const CreatureDev* __restrict__ descs, int gout, const float2* __r
 // l, l + 64, ... (Kahan), then the lanes combine in a fixed xor butterfly with TwoSum, so the
 // result is the same on every run.  gparam = [mu | sigma | w] then the compensation terms.
 // ---------------------------------------------------------------------------------------
-constexpr int kReduceWaves = 4;
-
-__global__ void __launch_bounds__(64 * kReduceWaves)
-k_param_reduce(const CreatureDev* __restrict__ descs, int nblk) {
-  const CreatureDev& d = descs[blockIdx.z];
-  const int K = d.K;
-  const int idx = (int)blockIdx.x * kReduceWaves + (int)(threadIdx.x >> 6);
-  if (idx >= 3 * K) return;  // a whole wavefront; the kernel has no barriers
-  const int lane = threadIdx.x & 63;
-  const int k = idx / 3, j = idx % 3;
-  gptr<const float> part = to_global((const float*)d.partial);
-  float s = 0.f, comp = 0.f;  // running sum and Kahan compensation (true sum = s - comp)
-  for (int b = lane; b < nblk; b += 64) {
-    const float yv = part[((size_t)b * K + k) * 3 + j] - comp;
-    const float tt = s + yv;
-    comp = (tt - s) - yv;
-    s = tt;
-  }
-  for (int m = 32; m >= 1; m >>= 1) {
-    const float s2 = __shfl_xor(s, m, 64), c2 = __shfl_xor(comp, m, 64);
-    const float t = s + s2;  // TwoSum: s + s2 = t + e exactly
-    const float bv = t - s;
-    const float e = (s - (t - bv)) + (s2 - bv);
-    comp = (comp + c2) - e;
-    s = t;
-  }
-  if (lane == 0) {
-    gptr<float> acc = to_global(d.gparam) + j * K + k;
-    gptr<float> cmp = to_global(d.gparam) + 3 * K + j * K + k;
-    const float yv = (s - comp) - *cmp;
-    const float tt = *acc + yv;
-    *cmp = (tt - *acc) - yv;
-    *acc = tt;
-  }
-}
 
 __global__ void __launch_bounds__(256) k_zero_gparam(const CreatureDev* __restrict__ descs) {
   const CreatureDev& d = descs[blockIdx.z];
@@ -492,13 +507,16 @@ static hipError_t launch_col_adj_tc(const StepLaunch& sl, int tc, hipStream_t st
   }
 }
 
+// red_nblk > 0: the inverse-row launch also reduces the parameter partials (an extra y slice);
+// only for row geometries of whole wavefronts (RowCfg THREADS a multiple of 64).
 template <int N, int ER>
-static hipError_t launch_adj_rows(const StepLaunch& sl, int gout, hipStream_t st, bool growth) {
+static hipError_t launch_adj_rows(const StepLaunch& sl, int gout, hipStream_t st, bool growth, int red_nblk = 0) {
   using CF = RowCfg<N, ER>;
   if (growth) {
     k_growth_adj_row<N, ER><<<dim3(CF::BLOCKS, sl.max_k, sl.count), CF::THREADS, 0, st>>>(sl.descs, sl.tw, sl.islot);
   } else {
-    k_row_inv_add<N, ER><<<dim3(CF::BLOCKS, sl.max_c, sl.count), CF::THREADS, 0, st>>>(sl.descs, gout, sl.tw);
+    const int gy = sl.max_c + (red_nblk > 0 ? 1 : 0);
+    k_row_inv_add<N, ER><<<dim3(CF::BLOCKS, gy, sl.count), CF::THREADS, 0, st>>>(sl.descs, gout, sl.tw, red_nblk);
   }
   return hipSuccess;
 }
@@ -527,15 +545,20 @@ static hipError_t launch_adjoint_n(const StepLaunch& sl, int gin, int gout, hipS
   const int waves = kAdjThreads / 64;
   k_diffusion_adj<<<dim3((tiles_x * tiles_y + waves - 1) / waves, sl.max_c, sl.count), kAdjThreads, 0, st>>>(
       sl.descs, sl.sin, gin, N, sl.islot);
-  auto rows = [&](bool growth) {
+  auto rows = [&](bool growth, int red_nblk) {
     const int re = growth ? (sl.fwd_e ? sl.fwd_e : DefaultGeom<N>::FWD_E) : (sl.inv_e ? sl.inv_e : DefaultGeom<N>::INV_E);
     switch (re) {
-      case 8: return launch_adj_rows<N, 8>(sl, gout, st, growth);
-      case 16: return launch_adj_rows<N, 16>(sl, gout, st, growth);
+      case 8: return launch_adj_rows<N, 8>(sl, gout, st, growth, red_nblk);
+      case 16: return launch_adj_rows<N, 16>(sl, gout, st, growth, red_nblk);
       default: return hipErrorInvalidValue;
     }
   };
-  if ((e = rows(true)) != hipSuccess) return e;
+  // Partials of the growth adjoint: one per block of its row geometry (the plan's forward one).
+  const int fe = sl.fwd_e ? sl.fwd_e : DefaultGeom<N>::FWD_E;
+  const int pnblk = fe == 16 ? RowCfg<N, 16>::BLOCKS : RowCfg<N, 8>::BLOCKS;
+  const int ie = sl.inv_e ? sl.inv_e : DefaultGeom<N>::INV_E;
+  const bool fold = (ie == 16 ? RowCfg<N, 16>::THREADS : RowCfg<N, 8>::THREADS) % 64 == 0;
+  if ((e = rows(true, 0)) != hipSuccess) return e;
   // The adjoint column kernel keeps a per-source accumulator beside the transform, so it runs
   // with 8 values per thread (16 would spill), on the plan's tile width or the widest valid one
   // below it, at most 4 columns (measured: wider tiles leave too few blocks, e.g. 192 blocks of
@@ -552,13 +575,16 @@ static hipError_t launch_adjoint_n(const StepLaunch& sl, int gin, int gout, hipS
     }
   }
   if ((sl.seg_g == 0 || sl.seg_mixed) && (e = launch_col_adj_tc<N, 8>(sl, tc, st)) != hipSuccess) return e;
-  if ((e = rows(false)) != hipSuccess) return e;
   // Parameter-gradient reduction over the partials of the growth adjoint, whose row geometry
-  // (the plan's forward one) sets their block count; the inverse row geometry may differ.
-  switch (sl.fwd_e ? sl.fwd_e : DefaultGeom<N>::FWD_E) {
-    case 8: launch_param_reduce<N, 8>(sl, st); break;
-    case 16: launch_param_reduce<N, 16>(sl, st); break;
-    default: return hipErrorInvalidValue;
+  // (the plan's forward one) sets their block count; the inverse row geometry may differ.  It
+  // rides on the inverse-row launch where that one runs whole wavefronts.
+  if ((e = rows(false, fold ? pnblk : 0)) != hipSuccess) return e;
+  if (!fold) {
+    switch (fe) {
+      case 8: launch_param_reduce<N, 8>(sl, st); break;
+      case 16: launch_param_reduce<N, 16>(sl, st); break;
+      default: return hipErrorInvalidValue;
+    }
   }
   return hipGetLastError();
 }
