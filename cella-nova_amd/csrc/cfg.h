@@ -57,6 +57,10 @@ __device__ __forceinline__ int xcd_tile(int bx) {
 // 1024^2), the largest column-profile support radius it takes, and the segment length per grid
 // size (0: full-length column transforms only).
 constexpr int seg_tc(int N) { return N >= 1024 ? 16 : 32; }
+// Values per thread of the segment transforms: 8 at 1024^2, where 16 leaves two-wavefront
+// blocks too few to fill the CUs (column pass 17.6 -> 15.9 us); 16 elsewhere (8 was 2 % slower
+// for the 512^2 batched search and 5 % slower at 2048^2).
+constexpr int seg_e(int N) { return N == 1024 ? 8 : 16; }
 constexpr int kSegRcap = 64;
 constexpr int seg_m(int N) { return N == 2048 ? 256 : (N == 1024 || N == 512) ? 128 : 0; }
 // The largest support radius a segment takes: half of its rows stay valid outputs.

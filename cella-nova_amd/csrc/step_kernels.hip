@@ -792,7 +792,7 @@ static hipError_t launch_rows_e(const StepLaunch& sl, hipStream_t st, bool fwd) 
 
 template <int N>
 static hipError_t launch_col_segments(const StepLaunch& sl, hipStream_t st) {
-  constexpr int M = seg_m(N), E = 16;
+  constexpr int M = seg_m(N), E = seg_e(N);
   if constexpr (M == 0) {
     return hipErrorInvalidValue;
   } else {
