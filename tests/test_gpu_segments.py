@@ -165,3 +165,38 @@ def test_generated_creature_has_segments(cn):
     info = b.info()
     b.close()
     assert info["segments"] > 0 and info["segment_radius"] < 26
+
+
+def _conv_errors(cn, orc, cr, n):
+    eng = cn.make_engine(cr)
+    info = eng.info()
+    eng.iterate()
+    K = cr["num_kernels"]
+    u = eng.download_conv_buffer().reshape(K, n, n) / n**2
+    eng.close()
+    want = orc.conv_f64(cr["state"].astype(np.float64), cr["kernels_fft"], cr["src"])
+    return info, [orc.rel_linf(u[k], want[k]) for k in range(K)]
+
+
+def test_per_kernel_radii(cn, orc):
+    """Kernels of radii 4..30 in one creature: each table is truncated at its own support, the
+    segments overlap by the largest."""
+    n = 1024
+    radii = np.array([4, 9, 13, 30], np.uint32)
+    cr = cn.generate_creature(n, 2, 4, np.array([0, 1, 0, 1]), np.array([0, 0, 1, 1]), radii, patch=256, seed=3,
+                              creature=1)
+    info, errs = _conv_errors(cn, orc, cr, n)
+    R = _numpy_radius(cr["kernels_fft"], n)
+    assert info["segments"] > 0 and info["segment_radius"] == R and R < 30
+    assert info["segments"] == -(-n // (info["segment_length"] - 2 * R))
+    assert max(errs) <= 2e-6, errs
+
+
+def test_radius_beyond_the_cap_uses_full_length_transforms(cn, orc):
+    """A kernel whose column support exceeds a quarter of the segment length (32 rows at 1024^2)
+    keeps the full-length column pass."""
+    n = 1024
+    cr = cn.generate_creature(n, 1, 1, np.array([0]), np.array([0]), 40, patch=256, seed=4, creature=2)
+    info, errs = _conv_errors(cn, orc, cr, n)
+    assert info["segments"] == 0
+    assert max(errs) <= 2e-6, errs
