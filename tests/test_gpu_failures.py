@@ -69,6 +69,35 @@ def test_render_survives_each_injected_failure(cn):
     assert n >= 3  # the frame buffer, its descriptor upload and the capture at least
 
 
+def test_set_kernel_survives_each_injected_failure(cn):
+    """set_kernel at 512^2 also measures the kernel's column support and builds its segment table
+    (temporary buffers, three launches, a read-back): each checked call fails in turn, and a
+    retry leaves the engine as one that never failed — the same segments, device bytes and
+    step results."""
+    n = 512
+    cr = cn.baseline_creature(n, num_channels=2, radius=9, seed=35, patch=64)
+    kf = np.array(cr["kernels_fft"])
+    ref = cn.make_engine(cr)
+    ref.set_kernel(kf[1], 2)
+    want_info = ref.info()
+    ref.run_steps(3)
+    want = ref.download_all_channels()
+    ref.close()
+    assert want_info["segments"] > 0
+
+    def same(e):
+        info = e.info()
+        for key in ("segments", "segment_radius", "device_bytes"):
+            assert info[key] == want_info[key], key
+        e.run_steps(3)
+        assert np.array_equal(e.download_all_channels(), want)
+
+    buf = np.ascontiguousarray(kf[1].astype(np.complex64))
+    n_fail = _walk(cn, lambda: cn.make_engine(cr),
+                   lambda e: cn.lib().cn_set_kernel(e._h, 2, buf.ctypes.data, buf.size), same)
+    assert n_fail >= 8  # the uploads, temporaries, profile, sums, read-back and table at least
+
+
 def test_rollout_survives_each_injected_failure(cn):
     cr = cn.baseline_creature(32, radius=4, seed=32, patch=16)
     target = cn.baseline_creature(32, radius=4, seed=33, patch=16)["state"]
