@@ -21,6 +21,7 @@
 #include "fft.h"
 #include "launch.h"
 #include "realfft.h"
+#include "segio.h"
 #include "types.h"
 
 namespace cn {
@@ -305,23 +306,15 @@ k_col_adj_segments(const CreatureDev* __restrict__ descs, const float2* __restri
   float2 acc[E];
 #pragma unroll
   for (int i = 0; i < E; ++i) acc[i] = make_float2(0.f, 0.f);
+  using IO = SegIO<N, M, E>;
   for (int kk = kb; kk < ke; ++kk) {
     const int k = d.src_k[kk];
     t = launder(t);
-    const BufRes sp = make_res((const char*)d.specg + (size_t)k * kPlaneBytes, kPlaneBytes);
     const BufRes kr = make_res(d.kseg + (size_t)k * (MH + 1) * W, kTabBytes);
     float2 v[E];
     float hk[E];
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      const int y = (base + in_pos<M, E, false>(t, i)) & (N - 1);
-      v[i] = buf_load_c64(sp, (uint32_t)(y * HALF + x) * 8u, 0);
-    }
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      const int m = out_pos<M, E, false>(t, i), mm = m <= MH ? m : M - m;
-      hk[i] = buf_load_f32(kr, (uint32_t)(mm * W + x) * 4u, 0);
-    }
+    IO::load((const char*)d.specg + (size_t)k * kPlaneBytes, base, t, x, v);
+    IO::table(d.kseg + (size_t)k * (MH + 1) * W, t, x, hk);
     fft_run<M, E, TC, true, false, kTwN, 0, true>(v, lds, t, lane, twl);
     if (edge) {
       __syncthreads();
@@ -346,12 +339,7 @@ k_col_adj_segments(const CreatureDev* __restrict__ descs, const float2* __restri
     }
   }
   fft_run<M, E, TC, true, true, kTwN, 0, true>(acc, lds, t, lane, twl + TF::size);
-  const BufRes ao = make_res((const char*)d.adjspec + (size_t)c * kPlaneBytes, kPlaneBytes);
-#pragma unroll
-  for (int i = 0; i < E; ++i) {
-    const int q = out_pos<M, E, true>(t, i) - R;
-    if (q >= 0 && q < rows) buf_store_c64(acc[i], ao, (uint32_t)res_index(N, o0 + q, x) * 8u, 0);
-  }
+  IO::store((char*)d.adjspec + (size_t)c * kPlaneBytes, acc, t, x, o0, R, rows);
 }
 
 constexpr int kReduceWaves = 4;

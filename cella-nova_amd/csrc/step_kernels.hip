@@ -17,6 +17,7 @@
 #include "launch.h"
 #include "mass.h"
 #include "realfft.h"
+#include "segio.h"
 #include "types.h"
 
 namespace cn {
@@ -266,22 +267,11 @@ k_col_segments(const CreatureDev* __restrict__ descs, const float2* __restrict__
   int t = (int)threadIdx.x / TC;
   const int o0 = seg * N / G, rows = (seg + 1) * N / G - o0, base = o0 - R;
 
-  const BufRes sp = make_res((const char*)d.spec + (size_t)c * kPlaneBytes, kPlaneBytes);
+  using IO = SegIO<N, M, E>;
   float2 v[E];
-#pragma unroll
-  for (int i = 0; i < E; ++i) {
-    const int y = (base + in_pos<M, E, false>(t, i)) & (N - 1);
-    v[i] = buf_load_c64(sp, (uint32_t)(y * HALF + x) * 8u, 0);
-  }
+  IO::load((const char*)d.spec + (size_t)c * kPlaneBytes, base, t, x, v);
   // Table factor of slot i (frequency m = out_pos(t, i), row min(m, M - m)).
-  auto load_h = [&](int k, float* out) {
-    const BufRes kr = make_res(d.kseg + (size_t)k * (MH + 1) * W, kTabBytes);
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      const int m = out_pos<M, E, false>(t, i), mm = m <= MH ? m : M - m;
-      out[i] = buf_load_f32(kr, (uint32_t)(mm * W + x) * 4u, 0);
-    }
-  };
+  auto load_h = [&](int k, float* out) { IO::table(d.kseg + (size_t)k * (MH + 1) * W, t, x, out); };
   using TF = TwLds<M, E, false>;
   using TI = TwLds<M, E, true>;
   __shared__ float2 twl[TF::size + TI::size + 1];
@@ -322,12 +312,7 @@ k_col_segments(const CreatureDev* __restrict__ descs, const float2* __restrict__
     if (kk + 1 < ke) load_h(d.src_k[kk + 1], hk);
     __builtin_amdgcn_s_setprio(1);
     fft_run<M, E, TC, true, true, kTwN, 0, true>(w, lds, t, lane, twl + TF::size);
-    const BufRes uo = make_res((const char*)d.uhat + (size_t)k * kPlaneBytes, kPlaneBytes);
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      const int q = out_pos<M, E, true>(t, i) - R;
-      if (q >= 0 && q < rows) buf_store_c64(w[i], uo, (uint32_t)res_index(N, o0 + q, x) * 8u, 0);
-    }
+    IO::store((char*)d.uhat + (size_t)k * kPlaneBytes, w, t, x, o0, R, rows);
     __builtin_amdgcn_s_setprio(0);
   }
 }
