@@ -58,6 +58,33 @@ def test_config2_2048_c3k9_per_step(cn, orc):
     assert drift <= 1e-6
 
 
+def test_config2_2048_c3k9_1000_steps_sampled(cn, orc):
+    """The config's whole 1000-step run (graph replays between the samples): the teacher-forced
+    step against FP64 at eight points spread over the run, and the FP64 mass drift of the run."""
+    n, R = 2048, 26
+    cr = cn.baseline_creature(n, radius=R, seed=0)
+    eng = cn.make_engine(cr)
+    kf, src, dst, mu, sg, w = torch_ref.creature_tensors(cr, DEV)
+    m0 = eng.mass_fp64().sum()
+    errs, done = {}, 0
+    for s in (1, 2, 10, 100, 250, 500, 750, 1000):
+        if s - 1 > done:
+            eng.run_steps(s - 1 - done)
+        prev = torch.tensor(eng.download_all_channels().reshape(3, n, n), dtype=torch.float64, device=DEV)
+        eng.iterate()
+        got = torch.tensor(eng.download_all_channels().reshape(3, n, n), dtype=torch.float64, device=DEV)
+        want = torch_ref.step(prev, kf, src, dst, mu, sg, w)
+        errs[s] = float((got - want).abs().max() / want.abs().max())
+        done = s
+    drift = abs(eng.mass_fp64().sum() - m0) / m0
+    assert eng.info()["steps_done"] == 1000
+    eng.close()
+    print("2048^2 C3K9 R=26, 1000 steps, teacher-forced step vs FP64 at " +
+          ", ".join(f"{s}: {e:.2e}" for s, e in errs.items()) + f"; FP64 mass drift {drift:.2e}")
+    assert max(errs.values()) <= TOL
+    assert drift <= 1e-6
+
+
 CHECKPOINTS = (1, 10, 25, 50, 100)
 CPU_THREADS = max(1, min(16, os.cpu_count() or 1))
 
