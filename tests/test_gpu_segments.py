@@ -200,3 +200,41 @@ def test_radius_beyond_the_cap_uses_full_length_transforms(cn, orc):
     info, errs = _conv_errors(cn, orc, cr, n)
     assert info["segments"] == 0
     assert max(errs) <= 2e-6, errs
+
+
+@pytest.mark.parametrize("case", range(8))
+def test_random_creatures_step_against_fp64(cn, case):
+    """Seeded random creatures at 512^2 and 1024^2: 1-4 channels, random kernel maps (channels
+    that feed no kernel or receive none included), per-kernel radii 3-30, so that creatures with
+    and without segments, several segment counts and ragged kernel groups all occur.  One step
+    from the seed against the float64 restatement of the reference step (torch, on the GPU)."""
+    torch = pytest.importorskip("torch")
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+    try:
+        import torch_ref
+    finally:
+        sys.path.pop(0)
+    rng = np.random.default_rng(1000 + case)
+    n = int(rng.choice([512, 1024]))
+    C = int(rng.integers(1, 5))
+    K = int(rng.integers(1, 2 * C + 2))
+    src = rng.integers(0, C, K)
+    dst = rng.integers(0, C, K)
+    radii = rng.integers(3, 31, K).astype(np.uint32)
+    cr = cn.generate_creature(n, C, K, src, dst, radii, patch=min(n // 2, 256), seed=case, creature=case)
+    eng = cn.make_engine(cr)
+    info = eng.info()
+    eng.iterate()
+    got = eng.download_all_channels().reshape(C, n, n)
+    eng.close()
+    dev = "cuda:0"
+    kf, s_, d_, mu, sg, w = torch_ref.creature_tensors(cr, dev)
+    want = torch_ref.step(torch.tensor(cr["state"], dtype=torch.float64, device=dev), kf, s_, d_, mu, sg, w)
+    want = want.cpu().numpy()
+    err = float(np.abs(got - want).max() / np.abs(want).max())
+    print(f"case {case}: {n}^2 C{C} K{K} radii {radii.tolist()} segments {info['segments']} "
+          f"(radius {info['segment_radius']}): rel L-inf {err:.2e}")
+    assert err <= 1e-5
+
