@@ -3,8 +3,8 @@ full-length column transforms and the FP64 oracle.
 
 A kernel's column profile h(y, kx) = sum_ky Khat_h(ky, kx) e^{2 pi i ky y / N} has the kernel's
 support in y; the engine finds it from the stored spectrum (tail energy within (2^-22)^2 of the
-total, in FP64) and convolves each column on segments of 256 rows that overlap by twice that
-radius.  Checked here: the radius against the same criterion computed with numpy; the
+total, in FP64) and convolves each column on segments of 256 rows (2048^2) or 128 rows (1024^2,
+512^2) that overlap by twice that radius.  Checked here: the radius against the same criterion computed with numpy; the
 convolution results against the full-length path of the same creature (plan col_full) and
 against the FP64 oracle; a kernel without compact support (a wide Gaussian) falls back to the
 full-length path, and back again; a batch that mixes both kinds of creatures steps each of them
