@@ -238,3 +238,24 @@ def test_random_creatures_step_against_fp64(cn, case):
           f"(radius {info['segment_radius']}): rel L-inf {err:.2e}")
     assert err <= 1e-5
 
+
+
+@pytest.mark.parametrize("radii,segmented", [([62, 40, 8], True), ([70, 12], False)])
+def test_2048_radii_at_the_cap(cn, orc, radii, segmented):
+    """2048^2 kernels of radius up to 62 (support 60, inside the 64-row cap: 256-row segments
+    that overlap by 120 rows, 16 per column) take the segmented pass; a radius-70 kernel keeps
+    the full-length pass.  Every convolution result within 2e-6 of the FP64 oracle (measured
+    3.7-3.8e-7)."""
+    K = len(radii)
+    src, dst = np.array([0, 1, 0][:K]), np.array([1, 0, 0][:K])
+    cr = cn.generate_creature(N, 2, K, src, dst, np.array(radii, np.uint32), patch=256, seed=9, creature=5)
+    info, errs = _conv_errors(cn, orc, cr, N)
+    R = _numpy_radius(cr["kernels_fft"], N)
+    print(f"radii {radii}: support {R}, segments {info['segments']} (radius {info['segment_radius']}), "
+          f"rel L-inf vs FP64 {max(errs):.2e}")
+    if segmented:
+        assert R is not None and R <= 64 and info["segment_radius"] == R
+        assert info["segments"] == -(-N // (256 - 2 * R)) and info["segment_length"] == 256
+    else:
+        assert R is None and info["segments"] == 0
+    assert max(errs) <= 2e-6, errs
