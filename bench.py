@@ -442,6 +442,26 @@ def main():
     kms = eng.profile_kernels(prof_steps)
     info = eng.info()
     compact = info["compact_kernels"]
+    # The boundary's host buffers (mod.rs:216-263): every channel uploaded from host f64
+    # (upload_channel, cast to f32) and all channels downloaded as host f32. The timed run's rate
+    # with one upload and one download of the state added is reported beside `value`.
+    state64 = [np.ascontiguousarray(cr["state"][c], dtype=np.float64).reshape(-1) for c in range(C)]
+    io_reps = 5
+    t_io = time.perf_counter()
+    for _ in range(io_reps):
+        for c in range(C):
+            eng.upload_channel(state64[c], c)
+        eng.sync()
+    upload_s = (time.perf_counter() - t_io) / io_reps
+    t_io = time.perf_counter()
+    for _ in range(io_reps):
+        eng.download_all_channels()
+    download_s = (time.perf_counter() - t_io) / io_reps
+    host_io = {"upload_ms": round(upload_s * 1e3, 3), "download_ms": round(download_s * 1e3, 3),
+               "upload_bytes_host": C * n * n * 8, "download_bytes_host": C * n * n * 4,
+               "value_with_host_io": round(n * n * args.steps / (dt + upload_s + download_s), 1),
+               "note": f"the {args.steps}-step run with one upload (host f64) and one download (host f32) "
+                       "of the state added; pageable host buffers"}
     eng.close()
     bpc = algorithmic_bytes_per_cell(C, K, compact, seg_table_bytes_per_cell(info))
     per_kernel = {}
@@ -472,6 +492,7 @@ def main():
                           "c2c_model_note": "bytes of the full-complex model (36C+32K B/cell) per step time; this "
                                             "design moves fewer bytes, so above 1 means the model is beaten"},
         "mass_drift_rel_fp64": drift,
+        "host_io": host_io,
     }
     if not args.no_grad:
         # The second half of the headline metric: batched forward + adjoint creature search
