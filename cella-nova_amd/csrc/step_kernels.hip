@@ -534,7 +534,10 @@ k_row_inv_growth_pair(const CreatureDev* __restrict__ descs, const float2* __res
         const float2 a = xa[m], b = xb[m];
         const float2 g = make_float2(a.x - b.y, a.y + b.x);
         if constexpr (m == 0) {
-          v[i] = (t == 0) ? make_float2(a.x, b.x) : g;
+          // The rows' DC bins less mu: the transform then gives u - mu (a constant row
+          // spectrum at kx = 0 adds that constant to every output), one subtraction per row
+          // pair instead of one per output.
+          v[i] = (t == 0) ? make_float2(a.x - mu, b.x - mu) : g;
         } else {
           v[i] = g;
         }
@@ -556,10 +559,10 @@ k_row_inv_growth_pair(const CreatureDev* __restrict__ descs, const float2* __res
       const int x = out_pos<N, E, true>(t, i);
       if constexpr (WRITE_U) {
         gptr<float> uo = to_global(d.u + (((size_t)islot * d.K + k) * N + y0) * N);
-        uo[x] = v[i].x;
-        uo[N + x] = v[i].y;
+        uo[x] = v[i].x + mu;
+        uo[N + x] = v[i].y + mu;
       }
-      const float d0 = v[i].x - mu, d1 = v[i].y - mu;
+      const float d0 = v[i].x, d1 = v[i].y;  // u - mu
       acc[i].x = fmaf(w2, __builtin_amdgcn_exp2f(d0 * (d0 * cexp)), acc[i].x);
       acc[i].y = fmaf(w2, __builtin_amdgcn_exp2f(d1 * (d1 * cexp)), acc[i].y);
     }
