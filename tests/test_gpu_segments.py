@@ -205,8 +205,8 @@ def test_radius_beyond_the_cap_uses_full_length_transforms(cn, orc):
 @pytest.mark.parametrize("case", range(8))
 def test_random_creatures_step_against_fp64(cn, case):
     """Seeded random creatures at 512^2 and 1024^2: 1-4 channels, random kernel maps (channels
-    that feed no kernel or receive none included), per-kernel radii 3-30, so that creatures with
-    and without segments, several segment counts and ragged kernel groups all occur.  One step
+    that feed no kernel or receive none included), per-kernel radii 3-30 (all within the 32-row
+    cap), so that segment counts from 5 to 15 per column and ragged kernel groups occur.  One step
     from the seed against the float64 restatement of the reference step (torch, on the GPU)."""
     torch = pytest.importorskip("torch")
     import os
