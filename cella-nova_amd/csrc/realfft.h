@@ -121,9 +121,10 @@ __device__ __forceinline__ void c2r_load(gptr<const float2> X, int t, float2* xp
   }
 }
 
+// dc is subtracted from the row's DC bin, so that every real output comes out less dc.
 template <int N, int E, bool LT = false>
 __device__ __forceinline__ void c2r_form(const float2* xp, const float2* xq, int t, const float2* __restrict__ tw,
-                                         float2* v) {
+                                         float2* v, float dc = 0.f) {
   constexpr int L = N / 2;
   const float2 lo = LT ? tw[t & 31] : make_float2(1.f, 0.f);
   // Branch-free (see r2c_finish): the twiddle is loaded for every lane; only the slot with a
@@ -139,7 +140,7 @@ __device__ __forceinline__ void c2r_form(const float2* xp, const float2* xq, int
     const float2 o2 = c_mulc(dd, W);                                       // * W_N^{-p}
     const float2 g = make_float2(e2.x - o2.y, e2.y + o2.x);               // e2 + i*o2
     if constexpr (c == 0) {
-      const float2 g0 = make_float2(xp[i].x + xp[i].y, xp[i].x - xp[i].y);
+      const float2 g0 = make_float2(xp[i].x + xp[i].y - dc, xp[i].x - xp[i].y - dc);
       v[i] = (p == 0) ? g0 : g;
     } else {
       v[i] = g;

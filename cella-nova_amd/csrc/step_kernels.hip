@@ -375,7 +375,7 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
       if constexpr (!kPrefetch) c2r_load_own_ct<N, E>(uhat_plane(k), y, t, xp);
       __builtin_amdgcn_s_setprio(0);
       c2r_partner<N, E, ROWS>(xp, lds, t, lane, xq);
-      c2r_form<N, E, true>(xp, xq, t, twl + TI::size, v);
+      c2r_form<N, E, true>(xp, xq, t, twl + TI::size, v, mu);  // the transform yields u - mu
       __builtin_amdgcn_s_setprio(1);
       if (kPrefetch && kk + 1 < ke) c2r_load_own_ct<N, E>(uhat_plane(d.dst_k[kk + 1]), y, t, xp);
       __builtin_amdgcn_s_setprio(0);
@@ -383,11 +383,10 @@ k_row_inv_growth(const CreatureDev* __restrict__ descs, const float2* __restrict
 #pragma unroll
       for (int i = 0; i < E; ++i) {
         const int n = out_pos<L, E, true>(t, i);
-        const float u0 = v[i].x, u1 = v[i].y;  // u[2n], u[2n+1]
+        const float d0 = v[i].x, d1 = v[i].y;  // u[2n] - mu, u[2n+1] - mu
         if constexpr (WRITE_U) {
-          ((gptr<float2>)to_global(d.u + (((size_t)islot * d.K + k) * N + y) * N))[n] = v[i];
+          ((gptr<float2>)to_global(d.u + (((size_t)islot * d.K + k) * N + y) * N))[n] = make_float2(d0 + mu, d1 + mu);
         }
-        const float d0 = u0 - mu, d1 = u1 - mu;
         acc[i].x = fmaf(w2, __builtin_amdgcn_exp2f(d0 * (d0 * cexp)), acc[i].x);
         acc[i].y = fmaf(w2, __builtin_amdgcn_exp2f(d1 * (d1 * cexp)), acc[i].y);
       }
